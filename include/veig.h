@@ -1,0 +1,163 @@
+/* veig.h -- C ABI of the MI355X (gfx950) batched complex eigensolver.
+ *
+ * Drop-in for the hot path of the reference header library: the batched
+ * eigendecomposition zgehrd -> zunghr -> zlahqr -> ztrevc of equal-size complex
+ * n x n matrices held in the reference's tiled split-plane layout.
+ *
+ * Layout (the reference's "vectorized data structure", utils.cpp:85-157):
+ *   T = thread_cnt, matrix m = b*T + t (b < block_cnt, t < T)
+ *   matrix element (r,c), 0-based, column-major:  b*T*n*n + (r + c*n)*T + t
+ *   vector element r (eigenvalues, tau):           b*T*n   + r*T       + t
+ * Real and imaginary parts live in separate double planes (rX, iX).
+ *
+ * Two families of entry points:
+ *   veig_*       device pointers, enqueued on `stream` (a hipStream_t; NULL =
+ *                the default stream), asynchronous, no allocation except where
+ *                stated.
+ *   veig_host_*  host pointers with the reference's exact argument lists
+ *                (utils.h:37-65); they allocate device memory, copy in, run the
+ *                device entry and copy back, like utils.cpp:631-692, and return
+ *                after the results are on the host.
+ * Every function returns a veig_status; veig_last_error() describes the last
+ * failure of the calling thread.  Sizes are 64-bit (the reference's 32-bit
+ * int byte counts overflow at 2^31 bytes per plane).
+ */
+#ifndef VEIG_H_
+#define VEIG_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  VEIG_OK = 0,
+  VEIG_ERR_INVALID = 1,     /* bad argument (negative size, NULL pointer, ...) */
+  VEIG_ERR_UNSUPPORTED = 2, /* order n beyond this build (n > 32 for the lane kernels) */
+  VEIG_ERR_HIP = 3,         /* HIP runtime error (launch, copy, allocation) */
+  VEIG_ERR_NOMEM = 4        /* device allocation failed */
+} veig_status;
+
+/* flags for veig_zgeev */
+#define VEIG_WRITE_SCHUR 1u /* also store the Schur form T over A (zeros below the diagonal) */
+
+/* Largest order handled by the lane-group kernels. */
+#define VEIG_MAX_DIM 32
+
+const char *veig_status_string(int status);
+const char *veig_last_error(void);
+/* Build/version string and the device the library runs on. */
+const char *veig_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Fused eigendecomposition, device pointers.
+ * Replaces eigensolver(dim, block_cnt, thread_cnt, rA, iA, rE, iE, rV, iV)
+ * (utils.cpp:578-698): E receives the eigenvalues in Schur (deflation) order,
+ * column k of V the right eigenvector Z*[x;1;0] of E(k) (the reference's
+ * scaling: the k-th entry in the Schur basis is 1, not normalised,
+ * Toolbox.hpp:1272-1275).  A is read only, unless VEIG_WRITE_SCHUR.
+ * info (may be NULL): per matrix, the number of eigenvalues accepted after
+ * `itmax` sweeps without a negligible subdiagonal (the reference caps at 30,
+ * Toolbox.hpp:808, and reports nothing; itmax <= 0 selects 30).
+ * ------------------------------------------------------------------------- */
+int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
+               double *rE, double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
+               uint32_t flags, void *stream);
+
+/* Same with the reduction, QR and eigenvector stages in complex64 (inputs
+ * and outputs stay complex128).  No refinement: the building block of the
+ * mixed-precision path, also usable on its own. */
+int veig_zgeev_fp32(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA,
+                    const double *iA, double *rE, double *iE, double *rV, double *iV,
+                    int32_t *info, int32_t itmax, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Stage entries, device pointers (the batch kernels of Toolbox-def.hpp:5-22).
+ * ------------------------------------------------------------------------- */
+/* test_zgehrd (Toolbox.hpp:2150) / call_zgehrd (utils.cpp:330): A -> upper
+ * Hessenberg H on and above the subdiagonal, reflector vectors v(i+2:n) below
+ * it, tau(1:n-1) (LAPACK zgehd2 convention; LAPACK zlarfg sign). */
+int veig_zgehrd(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
+                double *rtau, double *itau, void *stream);
+/* test_zunghr (Toolbox.hpp:2179) / call_zunghr (utils.cpp:376): reflectors in
+ * A (as left by veig_zgehrd) and tau -> the unitary Q, in place in A. */
+int veig_zunghr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
+                const double *rtau, const double *itau, void *stream);
+/* hessenberg (utils.cpp:417): A -> H (with reflectors, as zgehrd) and Q. */
+int veig_hessenberg(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
+                    double *rQ, double *iQ, void *stream);
+/* zlahqr_new (Toolbox.hpp:1984): upper Hessenberg H (entries below the
+ * subdiagonal are ignored) -> Schur form T in place (zeros below the
+ * diagonal), Z <- Z * (Schur vectors of H), E = diag(T). */
+int veig_zlahqr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rE, double *iE,
+                double *rH, double *iH, double *rZ, double *iZ, int32_t *info, int32_t itmax,
+                void *stream);
+/* test_ztrevc (Toolbox.hpp:2060) / call_ztrevc (utils.cpp:279): upper
+ * triangular T (read only), R <- R * X with X the unit upper triangular
+ * eigenvector matrix of T (column k of R*X is the eigenvector of T(k,k)). */
+int veig_ztrevc(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rT,
+                const double *iT, double *rR, double *iR, void *stream);
+/* test_zgemv (Toolbox.hpp:2118): y <- beta*y + A*x, or beta*y + A^H*x. */
+int veig_zgemv(int hermitian, int64_t dim, int64_t block_cnt, int64_t thread_cnt,
+               const double *rM, const double *iM, const double *rx, const double *ix,
+               double *ry, double *iy, double rbeta, double ibeta, void *stream);
+/* test_backsubstitution (Toolbox.hpp:2090): solve T x = b (T upper
+ * triangular), x over b in place. */
+int veig_backsubstitution(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rT,
+                          const double *iT, double *rb, double *ib, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Layout helpers on the device (utils.cpp:85-157).  `in` and `out` must not
+ * overlap.  AoS = one column-major n x n matrix after the other.
+ * ------------------------------------------------------------------------- */
+/* rearrangeMatrices (utils.cpp:85): AoS -> tiled SoA, one plane. */
+int veig_rearrange_matrices(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *in,
+                            double *out, void *stream);
+/* arrangeMatricesAoS (utils.cpp:105): tiled SoA -> AoS, one plane. */
+int veig_arrange_matrices_aos(int64_t dim, int64_t block_cnt, int64_t thread_cnt,
+                              const double *in, double *out, void *stream);
+/* repMatrixSoA (utils.cpp:125): one column-major n x n plane (device) copied
+ * into every batch slot. */
+int veig_rep_matrix_soa(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *matrix,
+                        double *out, void *stream);
+/* repVectorSoA (utils.cpp:145). */
+int veig_rep_vector_soa(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *vec,
+                        double *out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Host-pointer mirrors of the reference host API (utils.h:37-65; the first
+ * two are the Cython binding's externs, python_interface.pyx:12-28).
+ * ------------------------------------------------------------------------- */
+/* eigensolver(dim, block_cnt, thread_cnt, rA, iA, rE, iE, rV, iV), utils.cpp:578 */
+int veig_host_eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA,
+                          double *rE, double *iE, double *rV, double *iV);
+/* call_zgehrd, utils.cpp:330 */
+int veig_host_zgehrd(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rtau,
+                     double *itau);
+/* call_zunghr, utils.cpp:376 */
+int veig_host_zunghr(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rtau,
+                     double *itau);
+/* hessenberg, utils.cpp:417 */
+int veig_host_hessenberg(int dim, int block_cnt, int thread_cnt, double *rA, double *iA,
+                         double *rQ, double *iQ);
+/* call_zlahqr_new, utils.cpp:159.  As in the reference, S is overwritten with
+ * the initial Schur-vector matrix before the QR stage; the reference's
+ * argument swap at utils.cpp:169 makes that i*I, reproduced here so results
+ * match (pass the fixed-Z variant veig_zlahqr for identity). */
+int veig_host_zlahqr_new(int dim, int block_cnt, int thread_cnt, double *rE, double *iE,
+                         double *rM, double *iM, double *rS, double *iS);
+/* call_ztrevc, utils.cpp:279 */
+int veig_host_ztrevc(int dim, int block_cnt, int thread_cnt, double *rT, double *iT, double *rR,
+                     double *iR);
+/* call_zgemv, utils.cpp:218 */
+int veig_host_zgemv(int hermitian, int dim, int block_cnt, int thread_cnt, double *rM, double *iM,
+                    double *rx, double *ix, double *ry, double *iy, double rbeta, double ibeta);
+/* call_backsubstitution, utils.cpp:269 (declared in python_interface.pyx:24) */
+int veig_host_backsubstitution(int dim, int block_cnt, int thread_cnt, double *rT, double *iT,
+                               double *rb, double *ib);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VEIG_H_ */

@@ -1,0 +1,524 @@
+// Device-side building blocks of the batched complex eigensolver for gfx950.
+//
+// Execution model ("lane groups"): a wavefront of 64 lanes holds 64/NB
+// matrices of order n <= NB; lane r of a group owns index r of its matrix:
+//   * row r of the unitary factor (Q, then the Schur vectors Z, then V) in
+//     registers, as NB complex values z[0..NB-1];
+//   * in the reflector updates, row r (updates from the right) and column r
+//     (updates from the left) of the Hessenberg / Schur matrix, which lives in
+//     LDS as an NB x (NB+1) complex column-major tile (the +1 pad keeps both
+//     row-wise and column-wise walks free of bank conflicts).
+// Lanes of one group cooperate through LDS within a single wavefront, so the
+// only synchronisation needed is a wavefront-scope fence (LDS operations of a
+// wavefront complete in order).
+//
+// The arithmetic follows the reference chain zgehrd -> zunghr -> zlahqr ->
+// ztrevc (Toolbox.hpp:1315, 1492, 690, 1133) with these MI355X-first changes:
+//   * Q is accumulated forward (Q <- Q*H_i) in registers while the
+//     Hessenberg reduction runs, instead of a second pass over the stored
+//     reflectors (zunghr); the product is the same unitary matrix.
+//   * zgehrd's reflectors use LAPACK zlarfg's sign and its x == 0 guard
+//     (the reference's opposite sign, Toolbox.hpp:1367-1373, cancels and its
+//     missing guard divides by zero on triangular columns).
+//   * the entries below the subdiagonal are zeroed as the reduction goes, so
+//     the QR stage sees a clean Hessenberg matrix (the reference leaves the
+//     reflectors there, Toolbox.hpp:721-722, which corrupts its first sweep).
+//   * the deflation scan of every subdiagonal entry runs lane-parallel and is
+//     combined with one ballot.
+//   * the triangular solves for all eigenvectors run lane-parallel (lane k
+//     solves for eigenvector k) and the back-transform V = Z*X runs
+//     lane-parallel over rows of Z.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+namespace veig {
+
+// --------------------------------------------------------------------------
+// scalar complex arithmetic (split re/im, as ComplexDouble.hpp:283-488)
+// --------------------------------------------------------------------------
+template <typename R>
+struct cx {
+  R re, im;
+};
+
+template <typename R> struct lds2;
+template <> struct lds2<double> { using type = double2; };
+template <> struct lds2<float> { using type = float2; };
+
+template <typename R> struct limits;
+template <> struct limits<double> {
+  static constexpr double eps = DBL_EPSILON;
+  static constexpr double tiny = DBL_MIN;
+};
+template <> struct limits<float> {
+  static constexpr float eps = FLT_EPSILON;
+  static constexpr float tiny = FLT_MIN;
+};
+
+template <typename R> __device__ __forceinline__ cx<R> mk(R a, R b) { return cx<R>{a, b}; }
+template <typename R> __device__ __forceinline__ cx<R> add(cx<R> a, cx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R> __device__ __forceinline__ cx<R> sub(cx<R> a, cx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R> __device__ __forceinline__ cx<R> mul(cx<R> a, cx<R> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// a * conj(b)
+template <typename R> __device__ __forceinline__ cx<R> mulc(cx<R> a, cx<R> b) {
+  return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+// conj(a) * b
+template <typename R> __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+  return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+template <typename R> __device__ __forceinline__ cx<R> scale(cx<R> a, R s) { return {a.re * s, a.im * s}; }
+template <typename R> __device__ __forceinline__ cx<R> conj(cx<R> a) { return {a.re, -a.im}; }
+template <typename R> __device__ __forceinline__ R cabs1(cx<R> a) { return fabs(a.re) + fabs(a.im); }
+template <typename R> __device__ __forceinline__ R abs2(cx<R> a) { return a.re * a.re + a.im * a.im; }
+// 1/a as conj(a)/|a|^2 (one real division)
+template <typename R> __device__ __forceinline__ cx<R> inv(cx<R> a) {
+  const R s = R(1) / abs2(a);
+  return {a.re * s, -a.im * s};
+}
+// a/b by Smith's method (ComplexDouble.hpp:319-353), robust to |b| range
+template <typename R> __device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) {
+  if (fabs(b.re) <= fabs(b.im)) {
+    const R ratio = b.re / b.im, den = b.im * (R(1) + ratio * ratio);
+    return {(a.re * ratio + a.im) / den, (a.im * ratio - a.re) / den};
+  } else {
+    const R ratio = b.im / b.re, den = b.re * (R(1) + ratio * ratio);
+    return {(a.re + a.im * ratio) / den, (a.im - a.re * ratio) / den};
+  }
+}
+// principal square root (ComplexDouble.hpp:451-488)
+template <typename R> __device__ __forceinline__ cx<R> csqrt(cx<R> a) {
+  const R zr = sqrt(R(0.5) * (sqrt(abs2(a)) + fabs(a.re)));
+  const R zi = (zr == R(0)) ? R(0) : fabs(a.im) / (R(2) * zr);
+  cx<R> o = (a.re < R(0)) ? mk(zi, zr) : mk(zr, zi);
+  if (a.im < R(0)) o.im = -o.im;
+  return o;
+}
+
+template <typename R>
+__device__ __forceinline__ cx<R> ld(const typename lds2<R>::type* p) {
+  const typename lds2<R>::type v = *p;
+  return {v.x, v.y};
+}
+template <typename R>
+__device__ __forceinline__ void st(typename lds2<R>::type* p, cx<R> a) {
+  typename lds2<R>::type v;
+  v.x = a.re;
+  v.y = a.im;
+  *p = v;
+}
+
+// Wavefront-scope ordering of LDS traffic between lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum over the NB lanes of a lane group (butterfly, all lanes get the total).
+template <int NB, typename R>
+__device__ __forceinline__ R group_sum(R x) {
+#pragma unroll
+  for (int o = NB / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, NB);
+  return x;
+}
+
+// --------------------------------------------------------------------------
+// per-group context
+// --------------------------------------------------------------------------
+template <typename R, int NB>
+struct Group {
+  static constexpr int LD = NB + 1;
+  static constexpr int MPW = 64 / NB;  // matrices per wavefront
+  using L2 = typename lds2<R>::type;
+  L2* H;     // NB x LD complex tile, H(r,c) at H[c*LD + r]
+  L2* vsh;   // NB complex scratch (reflector vector)
+  int r;     // lane index within the group (row / column owned)
+  int mi;    // group index within the wavefront
+  int n;     // order of this group's matrix (0 if the slot is empty)
+  __device__ __forceinline__ L2* at(int row, int col) const { return H + col * LD + row; }
+};
+
+// --------------------------------------------------------------------------
+// Hessenberg reduction with forward accumulation of Q (zgehrd + zunghr,
+// Toolbox.hpp:1315-1659).  On exit H holds the upper Hessenberg matrix with
+// a real subdiagonal and exact zeros below it, and z[] holds row r of Q with
+// A = Q H Q^H.  nmax is the wavefront-uniform loop bound (max n in the wave).
+// If keep_reflectors, v(i+2:n) is left in H(i+2:n, i) and tau(i) is returned
+// in tau_out[i] of lane i (zgehrd's output convention).
+// --------------------------------------------------------------------------
+template <typename R, int NB>
+__device__ __forceinline__ void gehrd_q(const Group<R, NB>& g, cx<R> (&z)[NB], int nmax,
+                                        bool accumulate_q, bool keep_reflectors, cx<R>* tau_out) {
+  const int r = g.r, n = g.n;
+  for (int i = 0; i + 1 < nmax; ++i) {
+    // ---- reflector annihilating H(i+2:n, i) (zlarfg, LAPACK sign) ----
+    const bool in = r < n;
+    const cx<R> hri = in ? ld<R>(g.at(r, i)) : mk<R>(0, 0);
+    const R part = (r >= i + 2 && in) ? abs2(hri) : R(0);
+    const R xnorm2 = group_sum<NB>(part);
+    const bool live = i + 1 < n;  // this matrix still has a column to reduce
+    const cx<R> alpha = live ? ld<R>(g.at(i + 1, i)) : mk<R>(0, 0);
+    cx<R> tau = mk<R>(0, 0), scal = mk<R>(0, 0);
+    R beta = alpha.re;
+    if (live && !(xnorm2 == R(0) && alpha.im == R(0))) {
+      const R nrm = sqrt(abs2(alpha) + xnorm2);
+      beta = alpha.re >= R(0) ? -nrm : nrm;
+      const R ib = R(1) / beta;
+      tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
+      scal = inv(mk(alpha.re - beta, alpha.im));
+    }
+    cx<R> vr = mk<R>(0, 0);
+    if (live) {
+      if (r == i + 1) vr = mk<R>(1, 0);
+      else if (r >= i + 2 && in) vr = mul(hri, scal);
+    }
+    st<R>(g.vsh + r, vr);
+    if (live && in && r >= i + 1) {
+      if (r == i + 1) st<R>(g.at(r, i), mk<R>(beta, 0));
+      else st<R>(g.at(r, i), keep_reflectors ? vr : mk<R>(0, 0));
+    }
+    if (keep_reflectors && r == i) tau_out[0] = tau;
+    wave_sync();
+    // ---- H <- H * (I - tau v v^H) on rows 0..n-1, columns i+1..n-1 ----
+    if (live && in) {
+      cx<R> w = mk<R>(0, 0);
+      for (int c = i + 1; c < n; ++c) w = add(w, mul(ld<R>(g.at(r, c)), ld<R>(g.vsh + c)));
+      const cx<R> tw = mul(tau, w);
+      for (int c = i + 1; c < n; ++c) {
+        const cx<R> v = ld<R>(g.vsh + c);
+        st<R>(g.at(r, c), sub(ld<R>(g.at(r, c)), mulc(tw, v)));
+      }
+    }
+    // ---- Q <- Q * (I - tau v v^H) (row r of Q in registers) ----
+    if (accumulate_q) {
+      cx<R> v[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) v[c] = ld<R>(g.vsh + c);
+      cx<R> w = mk<R>(0, 0);
+#pragma unroll
+      for (int c = 0; c < NB; ++c) w = add(w, mul(z[c], v[c]));
+      const cx<R> tw = mul(tau, w);
+#pragma unroll
+      for (int c = 0; c < NB; ++c) z[c] = sub(z[c], mulc(tw, v[c]));
+    }
+    wave_sync();
+    // ---- H <- (I - conj(tau) v v^H) H on rows i+1..n-1, columns i+1..n-1 ----
+    if (live && in && r >= i + 1) {
+      cx<R> y = mk<R>(0, 0);
+      for (int k = i + 1; k < n; ++k) y = add(y, cmul(ld<R>(g.vsh + k), ld<R>(g.at(k, r))));
+      const cx<R> ty = cmul(tau, y);  // conj(tau) * y
+      for (int k = i + 1; k < n; ++k)
+        st<R>(g.at(k, r), sub(ld<R>(g.at(k, r)), mul(ld<R>(g.vsh + k), ty)));
+    }
+    wave_sync();
+  }
+}
+
+// --------------------------------------------------------------------------
+// 2-element Householder reflector of the bulge chase (zlarfg with n = 2,
+// LAPACK sign, Toolbox.hpp:183-242): given (a0, a1) returns t1, v2 and the
+// new a0 = beta (real) such that (I - t1 [1;v2][1;v2]^H)^H [a0;a1] = [beta;0].
+// --------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void larfg2(cx<R> a0, cx<R> a1, cx<R>& t1, cx<R>& v2, cx<R>& beta_out) {
+  if (a1.re == R(0) && a1.im == R(0) && a0.im == R(0)) {
+    t1 = mk<R>(0, 0);
+    v2 = a1;
+    beta_out = a0;
+    return;
+  }
+  const R nrm = sqrt(abs2(a0) + abs2(a1));
+  const R beta = a0.re >= R(0) ? nrm : -nrm;  // copysign(norm, Re a0)
+  const cx<R> zz = mk(a0.re + beta, a0.im);
+  const R ib = R(1) / beta;
+  t1 = scale(zz, ib);
+  v2 = mul(a1, inv(zz));
+  beta_out = mk<R>(-beta, 0);
+}
+
+// --------------------------------------------------------------------------
+// Single-shift complex QR to Schur form with Schur vectors (zlahqr_schur with
+// wantT = wantZ = 1, Toolbox.hpp:690-1031).  Deflation test (Ahues-Tisseur),
+// Wilkinson shift and exceptional shifts at iterations 10 and 20 as in the
+// reference; itmax sweeps per eigenvalue before it is accepted anyway (30 in
+// the reference, Toolbox.hpp:808); the count of such eigenvalues is returned.
+// If make_real_subdiag, the subdiagonal is first made real (loop 20,
+// Toolbox.hpp:735-793); after gehrd_q it already is.
+// --------------------------------------------------------------------------
+template <typename R, int NB>
+__device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int itmax,
+                                     bool make_real_subdiag) {
+  using L2 = typename Group<R, NB>::L2;
+  const int r = g.r, n = g.n;
+  const R ulp = limits<R>::eps;
+  const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
+
+  if (make_real_subdiag) {
+#pragma unroll
+    for (int i = 1; i < NB; ++i) {
+      cx<R> h = (i < n) ? ld<R>(g.at(i, i - 1)) : mk<R>(0, 0);
+      const bool doit = i < n && h.im != R(0);
+      cx<R> sc = mk<R>(1, 0);
+      if (doit) {
+        const R d1 = cabs1(h);
+        cx<R> w = scale(h, R(1) / d1);
+        const R d2 = sqrt(abs2(w));
+        sc = scale(conj(w), R(1) / d2);  // conj(h/|h|)
+      }
+      wave_sync();
+      if (doit) {
+        if (r == i) st<R>(g.at(i, i - 1), mk<R>(sqrt(abs2(h)), 0));
+        if (r >= i && r < n) st<R>(g.at(i, r), mul(ld<R>(g.at(i, r)), sc));
+        if (r <= i + 1 && r < n) st<R>(g.at(r, i), mulc(ld<R>(g.at(r, i)), sc));
+        z[i] = mulc(z[i], sc);
+      }
+      wave_sync();
+    }
+  }
+
+  int I = n - 1, L = 0, its = 0, nfail = 0;
+  bool done = (n <= 0);
+  const int base = g.mi * NB;
+  for (;;) {
+    // ---- bookkeeping until every matrix either needs a sweep or is done ----
+    bool ready = false;
+    for (;;) {
+      const bool need = !done && !ready;
+      if (!__any(need)) break;
+      bool neg = false;
+      if (need && its <= itmax) {
+        const int k = r;
+        if (k > L && k <= I) {
+          const cx<R> h = ld<R>(g.at(k, k - 1));
+          if (cabs1(h) <= smlnum) {
+            neg = true;
+          } else {
+            const cx<R> hkk1 = ld<R>(g.at(k - 1, k - 1)), hkk = ld<R>(g.at(k, k));
+            R tst = cabs1(hkk1) + cabs1(hkk);
+            if (tst == R(0)) {
+              if (k - 2 >= 0) tst += fabs(ld<R>(g.at(k - 1, k - 2)).re);
+              if (k + 1 <= n - 1) tst += fabs(ld<R>(g.at(k + 1, k)).re);
+            }
+            if (fabs(h.re) <= ulp * tst) {
+              const cx<R> hup = ld<R>(g.at(k - 1, k));
+              const R ab = fmax(cabs1(h), cabs1(hup)), ba = fmin(cabs1(h), cabs1(hup));
+              const R d3 = cabs1(sub(hkk1, hkk));
+              const R aa = fmax(cabs1(hkk), d3), bb = fmin(cabs1(hkk), d3);
+              const R s = aa + ab;
+              neg = ba * (ab / s) <= fmax(smlnum, ulp * (bb * (aa / s)));
+            }
+          }
+        }
+      }
+      const uint64_t bal = __ballot(neg);
+      if (need) {
+        if (its > itmax) {
+          // accepted without convergence (the reference's silent case)
+          ++nfail;
+          --I;
+          L = 0;
+          its = 0;
+          if (I < 0) done = true;
+        } else {
+          const uint32_t bits = (uint32_t)((bal >> base) & ((NB == 64) ? ~0ull : ((1ull << NB) - 1)));
+          if (bits) L = 31 - __clz(bits);
+          if (L > 0 && r == L) st<R>(g.at(L, L - 1), mk<R>(0, 0));
+          if (L >= I) {
+            --I;
+            L = 0;
+            its = 0;
+            if (I < 0) done = true;
+          } else {
+            ready = true;
+          }
+        }
+      }
+      wave_sync();
+    }
+    if (!__any(ready)) break;
+
+    // ---- shift and first column (Toolbox.hpp:896-1015) ----
+    cx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+    if (ready) {
+      cx<R> t;
+      if (its == 10) {
+        const R s = R(0.75) * fabs(ld<R>(g.at(L + 1, L)).re);
+        t = ld<R>(g.at(L, L));
+        t.re = s + t.re;
+      } else if (its == 20) {
+        const R s = R(0.75) * fabs(ld<R>(g.at(I, I - 1)).re);
+        t = ld<R>(g.at(I, I));
+        t.re = s + t.re;
+      } else {
+        t = ld<R>(g.at(I, I));
+        cx<R> u = mul(csqrt(ld<R>(g.at(I - 1, I))), csqrt(ld<R>(g.at(I, I - 1))));
+        R s = cabs1(u);
+        if (s != R(0)) {
+          cx<R> x = scale(sub(ld<R>(g.at(I - 1, I - 1)), t), R(0.5));
+          const R sx = cabs1(x);
+          s = fmax(s, sx);
+          const R is = R(1) / s;
+          const cx<R> xs = scale(x, is), us = scale(u, is);
+          cx<R> y = scale(csqrt(add(mul(xs, xs), mul(us, us))), s);
+          if (sx > R(0)) {
+            if (x.re * y.re + x.im * y.im < R(0)) y = mk(-y.re, -y.im);
+          }
+          t = sub(t, mul(cdiv(u, add(x, y)), u));
+        }
+      }
+      const cx<R> h11s = sub(ld<R>(g.at(L, L)), t);
+      const R h21 = ld<R>(g.at(L + 1, L)).re;
+      const R s = cabs1(h11s) + fabs(h21);
+      const R is = R(1) / s;
+      a0 = scale(h11s, is);
+      a1 = mk<R>(h21 * is, 0);
+    }
+
+    // ---- wavefront-uniform range of bulge positions ----
+    int kmin = NB, kmax = 0;
+#pragma unroll
+    for (int q = 0; q < Group<R, NB>::MPW; ++q) {
+      const int rq = __builtin_amdgcn_readlane(ready ? 1 : 0, q * NB);
+      const int lq = __builtin_amdgcn_readlane(L, q * NB);
+      const int iq = __builtin_amdgcn_readlane(I, q * NB);
+      if (rq) {
+        kmin = lq < kmin ? lq : kmin;
+        kmax = iq > kmax ? iq : kmax;
+      }
+    }
+
+    // ---- the sweep (Toolbox.hpp:138-335), K = bulge row ----
+#pragma unroll
+    for (int K = 0; K < NB - 1; ++K) {
+      if (K >= kmin && K < kmax) {
+        const bool act = ready && K >= L && K < I;
+        cx<R> b0 = a0, b1 = a1;
+        if (act && K > L) {
+          b0 = ld<R>(g.at(K, K - 1));
+          b1 = ld<R>(g.at(K + 1, K - 1));
+        }
+        cx<R> t1 = mk<R>(0, 0), v2 = mk<R>(0, 0), nb0 = b0;
+        if (act) larfg2(b0, b1, t1, v2, nb0);
+        const R t2 = t1.re * v2.re - t1.im * v2.im;  // Re(t1 * v2)
+        if (act && K > L) {
+          if (r == K) st<R>(g.at(K, K - 1), nb0);
+          if (r == K + 1) st<R>(g.at(K + 1, K - 1), mk<R>(0, 0));
+        }
+        // rows K, K+1 over columns K..n-1 (lane = column)
+        if (act && r >= K && r < n) {
+          L2* p1 = g.at(K, r);
+          L2* p2 = g.at(K + 1, r);
+          cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+          const cx<R> sum = add(cmul(t1, h1), scale(h2, t2));
+          h1 = sub(h1, sum);
+          h2 = sub(h2, mul(sum, v2));
+          st<R>(p1, h1);
+          st<R>(p2, h2);
+        }
+        wave_sync();
+        // columns K, K+1 over rows 0..min(K+2, I) (lane = row)
+        const int i3 = (K + 2) < I ? (K + 2) : I;
+        if (act && r <= i3) {
+          L2* p1 = g.at(r, K);
+          L2* p2 = g.at(r, K + 1);
+          cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+          const cx<R> sum = add(mul(t1, h1), scale(h2, t2));
+          h1 = sub(h1, sum);
+          h2 = sub(h2, mulc(sum, v2));
+          st<R>(p1, h1);
+          st<R>(p2, h2);
+        }
+        // Schur vectors: columns K, K+1 of row r
+        if (act) {
+          const cx<R> sum = add(mul(t1, z[K]), scale(z[K + 1], t2));
+          z[K] = sub(z[K], sum);
+          z[K + 1] = sub(z[K + 1], mulc(sum, v2));
+        }
+        wave_sync();
+      }
+    }
+
+    // ---- make H(I, I-1) real (Toolbox.hpp:337-367) ----
+    if (ready) {
+      const cx<R> h = ld<R>(g.at(I, I - 1));
+      if (h.im != R(0)) {
+        const R d = sqrt(abs2(h));
+        const R id = R(1) / d;
+        const cx<R> zz = mk(h.re * id, -h.im * id);  // conj(h/|h|)
+        if (r == I) st<R>(g.at(I, I - 1), mk<R>(d, 0));
+        if (r > I && r < n) st<R>(g.at(I, r), mul(ld<R>(g.at(I, r)), zz));
+        if (r < I) st<R>(g.at(r, I), mulc(ld<R>(g.at(r, I)), zz));
+#pragma unroll
+        for (int c = 1; c < NB; ++c)
+          if (c == I) z[c] = mulc(z[c], zz);
+      }
+      ++its;
+    }
+    wave_sync();
+  }
+  return nfail;
+}
+
+// --------------------------------------------------------------------------
+// Right eigenvectors of the upper-triangular T in LDS, back-transformed by the
+// Schur vectors (ztrevc + backsubstitution + zgemv, Toolbox.hpp:1133-1313,
+// 31-136, 1033-1131).  Lane k solves (T(0:k-1,0:k-1) - T(k,k) I) x = -T(0:k-1,k)
+// with the reference's small-pivot clamp; X (unit upper triangular) is written
+// over the strict upper triangle of T; then row r of V = Z*X is formed in
+// registers (column k of V = Z*[x; 1; 0], the reference's scaling).
+// On exit z[] holds row r of V and lam the eigenvalue T(r,r).
+// --------------------------------------------------------------------------
+template <typename R, int NB>
+__device__ __forceinline__ void trevc_backtransform(const Group<R, NB>& g, cx<R> (&z)[NB],
+                                                    cx<R>& lam, int nmax) {
+  const int r = g.r, n = g.n;
+  const R ulp = limits<R>::eps;
+  const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
+  const bool in = r < n;
+  lam = in ? ld<R>(g.at(r, r)) : mk<R>(0, 0);
+  const R smin = fmax(ulp * cabs1(lam), smlnum);
+  cx<R> x[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const bool a = in && j < r;
+    x[j] = a ? ld<R>(g.at(j, r)) : mk<R>(0, 0);
+    x[j] = mk(-x[j].re, -x[j].im);
+  }
+#pragma unroll
+  for (int j = NB - 2; j >= 0; --j) {
+    if (j + 1 < nmax) {
+      const bool act = in && j < r;
+      cx<R> d = sub(ld<R>(g.at(j, j)), lam);
+      if (cabs1(d) < smin) d = mk<R>(smin, 0);
+      const cx<R> xj = mul(x[j], inv(d));
+      if (act) x[j] = xj;
+#pragma unroll
+      for (int i = 0; i < j; ++i) {
+        const cx<R> tij = ld<R>(g.at(i, j));
+        if (act) x[i] = sub(x[i], mul(tij, xj));
+      }
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    if (in && j < r) st<R>(g.at(j, r), x[j]);
+  wave_sync();
+  // V(r, k) = z[k] + sum_{j<k} z[j] X(j,k); in place from the last column down
+#pragma unroll
+  for (int k = NB - 1; k >= 1; --k) {
+    if (k < nmax) {
+      cx<R> acc = z[k];
+#pragma unroll
+      for (int j = 0; j < k; ++j) acc = add(acc, mul(z[j], ld<R>(g.at(j, k))));
+      z[k] = acc;
+    }
+  }
+}
+
+}  // namespace veig

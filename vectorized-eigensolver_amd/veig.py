@@ -1,0 +1,224 @@
+"""ctypes binding of libveig.so (include/veig.h) and device-tensor helpers.
+
+The library is the product: every compute call here goes to the gfx950
+kernels.  There is no CPU fallback; if libveig.so is missing or the GPU is
+not usable the calls raise.
+
+Device entry points take torch tensors on the GPU (float64, contiguous, in
+the tiled split-plane layout of the reference, utils.cpp:85-157) and enqueue
+on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+REPO = PKG.parent
+LIB_PATH = PKG / "libveig.so"
+HEADER = REPO / "include" / "veig.h"
+
+VEIG_OK = 0
+VEIG_WRITE_SCHUR = 1
+MAX_DIM = 32
+
+
+class VeigError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.c_int64
+_i = ctypes.c_int
+_vp = ctypes.c_void_p
+
+# argument types of every function declared in include/veig.h
+_SIGS = {
+    "veig_status_string": ([_i], ctypes.c_char_p),
+    "veig_last_error": ([], ctypes.c_char_p),
+    "veig_version": ([], ctypes.c_char_p),
+    "veig_zgeev": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32,
+                    ctypes.c_uint32, _vp], _i),
+    "veig_zgeev_fp32": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp], _i),
+    "veig_zgeev_mixed": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                          ctypes.c_double, ctypes.c_int32, _vp, ctypes.c_size_t, _vp], _i),
+    "veig_mixed_workspace_size": ([_i64, _i64, _i64], ctypes.c_size_t),
+    "veig_zgehrd": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp], _i),
+    "veig_zunghr": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp], _i),
+    "veig_hessenberg": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp], _i),
+    "veig_zlahqr": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp], _i),
+    "veig_ztrevc": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp], _i),
+    "veig_zgemv": ([_i, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                    ctypes.c_double, _vp], _i),
+    "veig_backsubstitution": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp], _i),
+    "veig_rearrange_matrices": ([_i64, _i64, _i64, _vp, _vp, _vp], _i),
+    "veig_arrange_matrices_aos": ([_i64, _i64, _i64, _vp, _vp, _vp], _i),
+    "veig_rep_matrix_soa": ([_i64, _i64, _i64, _vp, _vp, _vp], _i),
+    "veig_rep_vector_soa": ([_i64, _i64, _i64, _vp, _vp, _vp], _i),
+    "veig_host_eigensolver": ([_i, _i, _i, _D, _D, _D, _D, _D, _D], _i),
+    "veig_host_zgehrd": ([_i, _i, _i, _D, _D, _D, _D], _i),
+    "veig_host_zunghr": ([_i, _i, _i, _D, _D, _D, _D], _i),
+    "veig_host_hessenberg": ([_i, _i, _i, _D, _D, _D, _D], _i),
+    "veig_host_zlahqr_new": ([_i, _i, _i, _D, _D, _D, _D, _D, _D], _i),
+    "veig_host_ztrevc": ([_i, _i, _i, _D, _D, _D, _D], _i),
+    "veig_host_zgemv": ([_i, _i, _i, _i, _D, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double], _i),
+    "veig_host_backsubstitution": ([_i, _i, _i, _D, _D, _D, _D], _i),
+    "veig_zgeev_varsize": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                            ctypes.c_int32, _vp], _i),
+}
+
+
+def header_functions() -> list[str]:
+    """Names of the functions declared in include/veig.h."""
+    txt = HEADER.read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(veig_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    """Load libveig.so (once).  Raises VeigError if it is not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise VeigError(f"{LIB_PATH} is not built; run vectorized-eigensolver_amd/build.py")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (argt, rest) in _SIGS.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.argtypes = argt
+                f.restype = rest
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != VEIG_OK:
+        L = lib()
+        raise VeigError(f"{what}: {L.veig_status_string(status).decode()} "
+                        f"({L.veig_last_error().decode()})")
+
+
+# --------------------------------------------------------------------------
+# device-tensor helpers (torch)
+# --------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    torch = _torch()
+    if not t.is_cuda:
+        raise VeigError("device entry points need GPU tensors")
+    if not t.is_contiguous():
+        raise VeigError("tensors must be contiguous")
+    if t.dtype not in (torch.float64, torch.int32, torch.uint8):
+        raise VeigError(f"unexpected dtype {t.dtype}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def zgeev(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, info=None,
+          itmax: int = 30, write_schur: bool = False, precision: str = "fp64", stream=None):
+    """Fused batched eigendecomposition on device tensors.
+
+    precision: "fp64" (veig_zgeev) or "fp32" (veig_zgeev_fp32, no refinement).
+    Returns (rE, iE, rV, iV)."""
+    torch = _torch()
+    nm = block_cnt * thread_cnt
+    dev = rA.device
+    if out is None:
+        rE = torch.empty(nm * dim, dtype=torch.float64, device=dev)
+        iE = torch.empty_like(rE)
+        rV = torch.empty(nm * dim * dim, dtype=torch.float64, device=dev)
+        iV = torch.empty_like(rV)
+    else:
+        rE, iE, rV, iV = out
+    L = lib()
+    if precision == "fp64":
+        st = L.veig_zgeev(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
+                          _ptr(rV), _ptr(iV), _ptr(info), itmax,
+                          VEIG_WRITE_SCHUR if write_schur else 0, _stream(stream))
+    elif precision == "fp32":
+        st = L.veig_zgeev_fp32(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
+                               _ptr(rV), _ptr(iV), _ptr(info), itmax, _stream(stream))
+    else:
+        raise ValueError(precision)
+    check(st, "veig_zgeev")
+    return rE, iE, rV, iV
+
+
+def mixed_workspace(dim: int, block_cnt: int, thread_cnt: int, device="cuda"):
+    torch = _torch()
+    nbytes = lib().veig_mixed_workspace_size(dim, block_cnt, thread_cnt)
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def zgeev_mixed(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, info=None,
+                berr=None, tol: float = 1e-12, itmax: int = 30, workspace=None, stream=None):
+    """Mixed precision: complex64 reduction/QR/eigenvectors, then FP64
+    residual-driven refinement of every eigenpair to backward error <= tol,
+    with an FP64 re-solve of the matrices that do not get there.
+    Returns (rE, iE, rV, iV)."""
+    torch = _torch()
+    nm = block_cnt * thread_cnt
+    dev = rA.device
+    if out is None:
+        rE = torch.empty(nm * dim, dtype=torch.float64, device=dev)
+        iE = torch.empty_like(rE)
+        rV = torch.empty(nm * dim * dim, dtype=torch.float64, device=dev)
+        iV = torch.empty_like(rV)
+    else:
+        rE, iE, rV, iV = out
+    if workspace is None:
+        workspace = mixed_workspace(dim, block_cnt, thread_cnt, dev)
+    st = lib().veig_zgeev_mixed(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
+                                _ptr(rV), _ptr(iV), _ptr(info), _ptr(berr), tol, itmax,
+                                _ptr(workspace), workspace.numel(), _stream(stream))
+    check(st, "veig_zgeev_mixed")
+    return rE, iE, rV, iV
+
+
+def _stage(name, *args, stream=None):
+    f = getattr(lib(), name)
+    conv = [(_ptr(a) if hasattr(a, "data_ptr") else a) for a in args]
+    check(f(*conv, _stream(stream)), name)
+
+
+def zgehrd(dim, bc, tc, rA, iA, rtau, itau, stream=None):
+    _stage("veig_zgehrd", dim, bc, tc, rA, iA, rtau, itau, stream=stream)
+
+
+def zunghr(dim, bc, tc, rA, iA, rtau, itau, stream=None):
+    _stage("veig_zunghr", dim, bc, tc, rA, iA, rtau, itau, stream=stream)
+
+
+def hessenberg(dim, bc, tc, rA, iA, rQ, iQ, stream=None):
+    _stage("veig_hessenberg", dim, bc, tc, rA, iA, rQ, iQ, stream=stream)
+
+
+def zlahqr(dim, bc, tc, rE, iE, rH, iH, rZ, iZ, info=None, itmax=30, stream=None):
+    f = lib().veig_zlahqr
+    check(f(dim, bc, tc, _ptr(rE), _ptr(iE), _ptr(rH), _ptr(iH), _ptr(rZ), _ptr(iZ), _ptr(info),
+            itmax, _stream(stream)), "veig_zlahqr")
+
+
+def ztrevc(dim, bc, tc, rT, iT, rR, iR, stream=None):
+    _stage("veig_ztrevc", dim, bc, tc, rT, iT, rR, iR, stream=stream)
+
+
+def layout(kind: str, dim, bc, tc, src, dst, stream=None):
+    name = {"soa": "veig_rearrange_matrices", "aos": "veig_arrange_matrices_aos",
+            "repm": "veig_rep_matrix_soa", "repv": "veig_rep_vector_soa"}[kind]
+    _stage(name, dim, bc, tc, src, dst, stream=stream)
