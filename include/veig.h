@@ -25,6 +25,7 @@
 #ifndef VEIG_H_
 #define VEIG_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -71,6 +72,26 @@ int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, d
 int veig_zgeev_fp32(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA,
                     const double *iA, double *rE, double *iE, double *rV, double *iV,
                     int32_t *info, int32_t itmax, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Mixed precision.  No counterpart in the reference (new in this design, for
+ * the same eigendecomposition as veig_zgeev): the reduction, QR sweeps and
+ * eigenvector solves run in complex64; every eigenpair is then refined in
+ * FP64 by Newton's method on R = A V - V diag(E) until the backward error
+ * ||R||_F / (||A||_F ||V||_F) <= tol (at most 3 corrections); matrices that do
+ * not get there, or whose complex64 QR did not converge, are solved again by
+ * the FP64 path.  A is read only.
+ *   info     (may be NULL): 0, or for matrices re-solved in FP64 that path's info
+ *   fallback (may be NULL): 1 for matrices re-solved in FP64, else 0
+ *   berr     (may be NULL): final backward error per matrix
+ *   workspace: device memory of at least veig_mixed_workspace_size() bytes,
+ *              256-byte aligned; contents need no initialisation.
+ * ------------------------------------------------------------------------- */
+size_t veig_mixed_workspace_size(int64_t dim, int64_t block_cnt, int64_t thread_cnt);
+int veig_zgeev_mixed(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA,
+                     const double *iA, double *rE, double *iE, double *rV, double *iV,
+                     int32_t *info, int32_t *fallback, double *berr, double tol, int32_t itmax,
+                     void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Stage entries, device pointers (the batch kernels of Toolbox-def.hpp:5-22).

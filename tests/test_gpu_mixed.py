@@ -1,0 +1,100 @@
+"""GPU tests of the mixed-precision path (veig_zgeev_mixed): complex64
+solve + FP64 Newton refinement + FP64 fallback, against the CPU oracle.
+
+Bar: every matrix ends with backward error ||AV - VE||_F/(||A||_F ||V||_F)
+<= 1e-12 (measured here independently of the kernel's own figure, which must
+agree), eigenvalues equal to the FP64 oracle's within 1e-10 * ||A||_F *
+cond, where cond is the eigenvector-matrix condition bound of the input
+family (1 for Gaussian/Hermitian, the construction's cond(V) otherwise).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import veig
+from helpers import backward_error, gaussian, match_eigenvalues, oracle_eig, to_soa, from_soa, vec_from_soa
+from test_gpu_parity import constructed, hermitian
+
+pytestmark = pytest.mark.gpu
+
+
+def run_mixed(A, tc=1, tol=1e-12):
+    N, n, _ = A.shape
+    rA, iA = to_soa(A, tc)
+    dev = torch.device("cuda:0")
+    trA, tiA = torch.from_numpy(rA).to(dev), torch.from_numpy(iA).to(dev)
+    info = torch.full((N,), -7, dtype=torch.int32, device=dev)
+    fb = torch.full((N,), -7, dtype=torch.int32, device=dev)
+    be = torch.full((N,), -1.0, dtype=torch.float64, device=dev)
+    rE, iE, rV, iV = veig.zgeev_mixed(trA, tiA, n, N // tc, tc, info=info, berr=be, fallback=fb, tol=tol)
+    torch.cuda.synchronize()
+    E = vec_from_soa(rE.cpu().numpy(), iE.cpu().numpy(), n, tc)
+    V = from_soa(rV.cpu().numpy(), iV.cpu().numpy(), n, tc)
+    return E, V, info.cpu().numpy(), fb.cpu().numpy(), be.cpu().numpy()
+
+
+def check(A, E, V, info, fb, be, tol, cond=1.0):
+    nrm = np.linalg.norm(A, axis=(1, 2))
+    my_be = backward_error(A, E, V)
+    assert np.all(np.isin(fb, [0, 1])), "fallback flags not written"
+    assert np.all(info == 0), f"info {np.unique(info)}"
+    assert np.all(my_be <= tol), f"backward error {my_be.max():.2e} > {tol}"
+    assert np.allclose(be, my_be, rtol=1e-3, atol=1e-16), "kernel backward error disagrees"
+    Eo, _, _ = oracle_eig(A, 1)
+    _, err = match_eigenvalues(Eo, E)
+    assert np.all(err <= 1e-10 * nrm * cond), f"eigenvalue error {np.max(err / nrm):.2e}"
+    return my_be
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 24, 32])
+def test_mixed_gaussian(gpu, n):
+    A = gaussian(128, n, 5000 + n)
+    E, V, info, fb, be = run_mixed(A)
+    check(A, E, V, info, fb, be, 1e-12)
+    assert fb.mean() < 0.05, f"fallback rate {fb.mean():.3f}"
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_mixed_hermitian(gpu, n):
+    A = hermitian(128, n, 5100 + n)
+    E, V, info, fb, be = run_mixed(A)
+    check(A, E, V, info, fb, be, 1e-12)
+
+
+@pytest.mark.parametrize("cond", [1e1, 1e3, 1e6])
+def test_mixed_conditioned(gpu, cond):
+    # configuration of the mixed-precision workload: A = V diag(lam) V^-1
+    # with cond(V) up to 1e6; matrices the refinement cannot bring to the
+    # tolerance fall back to FP64, so the bar holds for all of them
+    A = constructed(128, 24, 5200 + int(np.log10(cond)), cond=cond)
+    E, V, info, fb, be = run_mixed(A)
+    check(A, E, V, info, fb, be, 1e-12, cond=cond)
+
+
+def test_mixed_tolerance_and_tiles(gpu):
+    A = gaussian(256, 16, 5300)
+    for tc in (1, 4, 64):
+        E, V, info, fb, be = run_mixed(A, tc=tc, tol=1e-14)
+        check(A, E, V, info, fb, be, 1e-14)
+
+
+def test_fp32_solve_accuracy(gpu):
+    # the complex64 building block alone: single-precision accuracy
+    from gpu_util import gpu_eig
+    A = gaussian(128, 16, 5400)
+    E, V, info = gpu_eig(A, 1, precision="fp32")
+    be = backward_error(A, E, V)
+    assert np.all(info == 0)
+    assert be.max() < 1e-5 and be.max() > 1e-9, f"fp32 backward error {be.max():.2e}"
+
+
+def test_mixed_workspace_checks(gpu):
+    A = gaussian(8, 8, 1)
+    rA, iA = to_soa(A, 1)
+    dev = torch.device("cuda:0")
+    trA, tiA = torch.from_numpy(rA).to(dev), torch.from_numpy(iA).to(dev)
+    small = torch.empty(16, dtype=torch.uint8, device=dev)
+    with pytest.raises(veig.VeigError, match="workspace too small"):
+        veig.zgeev_mixed(trA, tiA, 8, 8, 1, workspace=small)

@@ -1,4 +1,4 @@
-// Batched complex eigensolver for MI355X (gfx950): kernels and C ABI.
+// Batched complex eigensolver for MI355X (gfx950): FP64 kernels and C ABI.
 // See include/veig.h for the interface and veig_device.hpp for the
 // lane-group execution model.
 #include <hip/hip_runtime.h>
@@ -8,189 +8,18 @@
 #include <cstring>
 #include <string>
 
-#include "veig_device.hpp"
-#include "../../include/veig.h"
+#include "veig_common.hpp"
+
+namespace veig {
+std::string &last_error() {
+  static thread_local std::string s;
+  return s;
+}
+}  // namespace veig
 
 using namespace veig;
 
 namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int status, const std::string &msg) {
-  g_last_error = msg;
-  return status;
-}
-
-int hip_fail(hipError_t e, const char *what) {
-  return fail(VEIG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define VEIG_HIP(call)                                  \
-  do {                                                  \
-    hipError_t e_ = (call);                             \
-    if (e_ != hipSuccess) return hip_fail(e_, #call);   \
-  } while (0)
-
-// Batch geometry shared by all kernels.
-struct Geom {
-  int64_t nmat;  // block_cnt * thread_cnt
-  int64_t T;     // thread_cnt (tile)
-  int n;         // order
-  __device__ __forceinline__ int64_t mat_base(int64_t m) const {
-    const int64_t b = m / T, t = m - b * T;
-    return b * T * (int64_t)n * n + t;
-  }
-  __device__ __forceinline__ int64_t vec_base(int64_t m, int mult = 1) const {
-    const int64_t b = m / T, t = m - b * T;
-    return (int64_t)mult * b * T * n + t;
-  }
-  __device__ __forceinline__ int64_t el(int r, int c) const { return ((int64_t)r + (int64_t)c * n) * T; }
-  __device__ __forceinline__ int64_t ve(int r) const { return (int64_t)r * T; }
-};
-
-int check_geom(int64_t dim, int64_t block_cnt, int64_t thread_cnt) {
-  if (dim < 0 || block_cnt < 0 || thread_cnt < 1)
-    return fail(VEIG_ERR_INVALID, "invalid batch geometry (dim < 0, block_cnt < 0 or thread_cnt < 1)");
-  if (dim > VEIG_MAX_DIM)
-    return fail(VEIG_ERR_UNSUPPORTED, "order " + std::to_string(dim) + " exceeds VEIG_MAX_DIM (" +
-                                          std::to_string(VEIG_MAX_DIM) + ")");
-  if (block_cnt > 0 && thread_cnt > INT64_MAX / block_cnt) return fail(VEIG_ERR_INVALID, "batch size overflows");
-  return VEIG_OK;
-}
-
-int nb_for(int64_t n) { return n <= 8 ? 8 : (n <= 16 ? 16 : 32); }
-
-// --------------------------------------------------------------------------
-// common kernel prologue: lane group setup and LDS tile
-// --------------------------------------------------------------------------
-template <typename R, int NB>
-struct Tiles {
-  using L2 = typename lds2<R>::type;
-  static constexpr int MPW = 64 / NB;
-  static constexpr int LD = NB + 1;
-  L2 H[MPW * NB * LD];
-  L2 v[MPW * NB];
-};
-
-template <typename R, int NB>
-__device__ __forceinline__ Group<R, NB> make_group(Tiles<R, NB> &sm, int n_of_matrix) {
-  Group<R, NB> g;
-  const int lane = threadIdx.x;
-  g.mi = lane / NB;
-  g.r = lane % NB;
-  g.n = n_of_matrix;
-  g.H = sm.H + g.mi * NB * Group<R, NB>::LD;
-  g.vsh = sm.v + g.mi * NB;
-  // zero the whole tile: entries outside the matrix (and below the
-  // subdiagonal when only a Hessenberg part is loaded) read as exact zeros
-#pragma unroll
-  for (int c = 0; c < NB; ++c) st<R>(g.at(g.r, c), mk<R>(0, 0));
-  st<R>(g.vsh + g.r, mk<R>(0, 0));
-  return g;
-}
-
-// lane r loads row r of a full matrix into the tile
-template <typename R, int NB>
-__device__ __forceinline__ void load_rows(const Group<R, NB> &g, const Geom &G, int64_t base,
-                                          const double *re, const double *im, int lower_band) {
-  if (g.r < g.n) {
-    for (int c = 0; c < g.n; ++c) {
-      if (g.r <= c + lower_band) {
-        const int64_t idx = base + G.el(g.r, c);
-        st<R>(g.at(g.r, c), mk<R>((R)re[idx], (R)im[idx]));
-      }
-    }
-  }
-}
-
-// lane r loads row r of a matrix into registers
-template <typename R, int NB>
-__device__ __forceinline__ void load_row_regs(const Group<R, NB> &g, const Geom &G, int64_t base,
-                                              const double *re, const double *im, cx<R> (&z)[NB]) {
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    z[c] = mk<R>(0, 0);
-    if (g.r < g.n && c < g.n) {
-      const int64_t idx = base + G.el(g.r, c);
-      z[c] = mk<R>((R)re[idx], (R)im[idx]);
-    }
-  }
-}
-
-template <typename R, int NB>
-__device__ __forceinline__ void store_row_regs(const Group<R, NB> &g, const Geom &G, int64_t base,
-                                               double *re, double *im, const cx<R> (&z)[NB]) {
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    if (g.r < g.n && c < g.n) {
-      const int64_t idx = base + G.el(g.r, c);
-      re[idx] = (double)z[c].re;
-      im[idx] = (double)z[c].im;
-    }
-  }
-}
-
-// lane r stores row r of the tile (full matrix)
-template <typename R, int NB>
-__device__ __forceinline__ void store_rows(const Group<R, NB> &g, const Geom &G, int64_t base,
-                                           double *re, double *im) {
-  if (g.r < g.n) {
-    for (int c = 0; c < g.n; ++c) {
-      const int64_t idx = base + G.el(g.r, c);
-      const cx<R> v = ld<R>(g.at(g.r, c));
-      re[idx] = (double)v.re;
-      im[idx] = (double)v.im;
-    }
-  }
-}
-
-template <typename R, int NB>
-__device__ __forceinline__ void identity_rows(const Group<R, NB> &g, cx<R> (&z)[NB]) {
-#pragma unroll
-  for (int c = 0; c < NB; ++c) z[c] = mk<R>(c == g.r ? R(1) : R(0), 0);
-}
-
-// --------------------------------------------------------------------------
-// fused eigendecomposition: A -> (E, V)
-// --------------------------------------------------------------------------
-struct FusedArgs {
-  double *rA, *iA;
-  double *rE, *iE, *rV, *iV;
-  int32_t *info;
-  Geom G;
-  int itmax;
-  uint32_t flags;
-};
-
-template <typename R, int NB>
-__global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
-  __shared__ Tiles<R, NB> sm;
-  constexpr int MPW = Tiles<R, NB>::MPW;
-  const int64_t m = (int64_t)blockIdx.x * MPW + threadIdx.x / NB;
-  const bool valid = m < a.G.nmat;
-  const int n = valid ? a.G.n : 0;
-  Group<R, NB> g = make_group<R, NB>(sm, n);
-  const int64_t base = valid ? a.G.mat_base(m) : 0;
-  const int64_t vbase = valid ? a.G.vec_base(m) : 0;
-  wave_sync();
-  load_rows<R, NB>(g, a.G, base, a.rA, a.iA, NB);
-  cx<R> z[NB];
-  identity_rows<R, NB>(g, z);
-  wave_sync();
-  const int nmax = a.G.n;  // every matrix of a launch has the same order
-  gehrd_q<R, NB>(g, z, nmax, true, false, nullptr);
-  const int nfail = lahqr<R, NB>(g, z, a.itmax, false);
-  if (a.flags & VEIG_WRITE_SCHUR) store_rows<R, NB>(g, a.G, base, a.rA, a.iA);
-  cx<R> lam;
-  trevc_backtransform<R, NB>(g, z, lam, nmax);
-  store_row_regs<R, NB>(g, a.G, base, a.rV, a.iV, z);
-  if (g.r < g.n) {
-    a.rE[vbase + a.G.ve(g.r)] = (double)lam.re;
-    a.iE[vbase + a.G.ve(g.r)] = (double)lam.im;
-  }
-  if (a.info && valid && g.r == 0) a.info[m] = nfail;
-}
 
 // --------------------------------------------------------------------------
 // stage kernels
@@ -212,7 +41,7 @@ __global__ __launch_bounds__(64) void k_stage(StageArgs a) {
   using R = double;
   __shared__ Tiles<R, NB> sm;
   constexpr int MPW = Tiles<R, NB>::MPW;
-  const int64_t m = (int64_t)blockIdx.x * MPW + threadIdx.x / NB;
+  const int64_t m = xcd_block() * MPW + threadIdx.x / NB;
   const bool valid = m < a.G.nmat;
   const int n = valid ? a.G.n : 0;
   Group<R, NB> g = make_group<R, NB>(sm, n);
@@ -352,21 +181,6 @@ __global__ void k_layout(int dir, Geom G, const double *in, double *out, int64_t
   }
 }
 
-template <typename R>
-int launch_zgeev(const FusedArgs &a, hipStream_t s) {
-  const int nb = nb_for(a.G.n);
-  const int mpw = 64 / nb;
-  const int64_t grid = (a.G.nmat + mpw - 1) / mpw;
-  if (grid > INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");
-  switch (nb) {
-    case 8: hipLaunchKernelGGL((k_zgeev<R, 8>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((k_zgeev<R, 16>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_zgeev<R, 32>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
-  }
-  VEIG_HIP(hipGetLastError());
-  return VEIG_OK;
-}
-
 int launch_stage(const StageArgs &a, hipStream_t s) {
   const int nb = nb_for(a.G.n);
   const int mpw = 64 / nb;
@@ -381,20 +195,7 @@ int launch_stage(const StageArgs &a, hipStream_t s) {
   return VEIG_OK;
 }
 
-Geom make_geom(int64_t dim, int64_t bc, int64_t tc) {
-  Geom G;
-  G.nmat = bc * tc;
-  G.T = tc;
-  G.n = (int)dim;
-  return G;
-}
 
-bool empty(int64_t dim, int64_t bc, int64_t tc) { return dim == 0 || bc == 0 || tc == 0; }
-
-#define REQUIRE(p)                                                                    \
-  do {                                                                                \
-    if (!(p)) return fail(VEIG_ERR_INVALID, std::string("NULL pointer argument: ") + #p); \
-  } while (0)
 
 // RAII device buffer for the host mirrors
 struct DevBuf {
@@ -419,12 +220,6 @@ int d2h(double *h, const DevBuf &b, size_t count) {
   return VEIG_OK;
 }
 
-#define TRY(x)                   \
-  do {                           \
-    int st_ = (x);               \
-    if (st_ != VEIG_OK) return st_; \
-  } while (0)
-
 }  // namespace
 
 // ============================================================================
@@ -443,64 +238,64 @@ const char *veig_status_string(int status) {
   }
 }
 
-const char *veig_last_error(void) { return g_last_error.c_str(); }
+const char *veig_last_error(void) { return last_error().c_str(); }
 
 const char *veig_version(void) { return "veig 0.1 gfx950 lane-group kernels (n <= 32)"; }
 
 int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
                double *rE, double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
                uint32_t flags, void *stream) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rA); REQUIRE(iA); REQUIRE(rE); REQUIRE(iE); REQUIRE(rV); REQUIRE(iV);
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV); VEIG_REQUIRE(iV);
   FusedArgs a{rA, iA, rE, iE, rV, iV, info, make_geom(dim, block_cnt, thread_cnt),
-              itmax > 0 ? itmax : 30, flags};
+              itmax > 0 ? itmax : 30, flags, nullptr, nullptr, nullptr};
   return launch_zgeev<double>(a, (hipStream_t)stream);
 }
 
 int veig_zgeev_fp32(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA,
                     const double *iA, double *rE, double *iE, double *rV, double *iV,
                     int32_t *info, int32_t itmax, void *stream) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rA); REQUIRE(iA); REQUIRE(rE); REQUIRE(iE); REQUIRE(rV); REQUIRE(iV);
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV); VEIG_REQUIRE(iV);
   FusedArgs a{const_cast<double *>(rA), const_cast<double *>(iA), rE, iE, rV, iV, info,
-              make_geom(dim, block_cnt, thread_cnt), itmax > 0 ? itmax : 30, 0u};
+              make_geom(dim, block_cnt, thread_cnt), itmax > 0 ? itmax : 30, 0u, nullptr, nullptr, nullptr};
   return launch_zgeev<float>(a, (hipStream_t)stream);
 }
 
 int veig_zgehrd(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
                 double *rtau, double *itau, void *stream) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rA); REQUIRE(iA); REQUIRE(rtau); REQUIRE(itau);
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rtau); VEIG_REQUIRE(itau);
   StageArgs a{rA, iA, nullptr, nullptr, rtau, itau, nullptr, make_geom(dim, block_cnt, thread_cnt), 30, ST_GEHRD};
   return launch_stage(a, (hipStream_t)stream);
 }
 
 int veig_hessenberg(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
                     double *rQ, double *iQ, void *stream) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rA); REQUIRE(iA); REQUIRE(rQ); REQUIRE(iQ);
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rQ); VEIG_REQUIRE(iQ);
   // tau is produced and consumed on chip; the launch still needs a place
   // for it, which is the sub-diagonal-free part of nothing: reuse a scratch
   // allocation (the only allocating device entry).
   const size_t sv = (size_t)dim * block_cnt * thread_cnt;
   DevBuf rt, it;
-  TRY(dev_alloc(rt, sv));
-  TRY(dev_alloc(it, sv));
+  VEIG_TRY(dev_alloc(rt, sv));
+  VEIG_TRY(dev_alloc(it, sv));
   StageArgs a{rA, iA, rQ, iQ, rt.p, it.p, nullptr, make_geom(dim, block_cnt, thread_cnt), 30, ST_HESS};
-  TRY(launch_stage(a, (hipStream_t)stream));
+  VEIG_TRY(launch_stage(a, (hipStream_t)stream));
   VEIG_HIP(hipStreamSynchronize((hipStream_t)stream));
   return VEIG_OK;
 }
 
 int veig_zunghr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
                 const double *rtau, const double *itau, void *stream) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rA); REQUIRE(iA); REQUIRE(rtau); REQUIRE(itau);
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rtau); VEIG_REQUIRE(itau);
   StageArgs a{rA, iA, nullptr, nullptr, const_cast<double *>(rtau), const_cast<double *>(itau), nullptr,
               make_geom(dim, block_cnt, thread_cnt), 30, ST_UNGHR};
   return launch_stage(a, (hipStream_t)stream);
@@ -509,9 +304,9 @@ int veig_zunghr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, 
 int veig_zlahqr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rE, double *iE,
                 double *rH, double *iH, double *rZ, double *iZ, int32_t *info, int32_t itmax,
                 void *stream) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rE); REQUIRE(iE); REQUIRE(rH); REQUIRE(iH); REQUIRE(rZ); REQUIRE(iZ);
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rH); VEIG_REQUIRE(iH); VEIG_REQUIRE(rZ); VEIG_REQUIRE(iZ);
   StageArgs a{rH, iH, rZ, iZ, rE, iE, info, make_geom(dim, block_cnt, thread_cnt), itmax > 0 ? itmax : 30,
               ST_LAHQR};
   return launch_stage(a, (hipStream_t)stream);
@@ -519,9 +314,9 @@ int veig_zlahqr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rE, 
 
 int veig_ztrevc(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rT,
                 const double *iT, double *rR, double *iR, void *stream) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rT); REQUIRE(iT); REQUIRE(rR); REQUIRE(iR);
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rT); VEIG_REQUIRE(iT); VEIG_REQUIRE(rR); VEIG_REQUIRE(iR);
   StageArgs a{const_cast<double *>(rT), const_cast<double *>(iT), rR, iR, nullptr, nullptr, nullptr,
               make_geom(dim, block_cnt, thread_cnt), 30, ST_TREVC};
   return launch_stage(a, (hipStream_t)stream);
@@ -531,8 +326,8 @@ int veig_zgemv(int hermitian, int64_t dim, int64_t block_cnt, int64_t thread_cnt
                const double *rM, const double *iM, const double *rx, const double *ix,
                double *ry, double *iy, double rbeta, double ibeta, void *stream) {
   if (dim < 0 || block_cnt < 0 || thread_cnt < 1) return fail(VEIG_ERR_INVALID, "invalid batch geometry");
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rM); REQUIRE(iM); REQUIRE(rx); REQUIRE(ix); REQUIRE(ry); REQUIRE(iy);
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rM); VEIG_REQUIRE(iM); VEIG_REQUIRE(rx); VEIG_REQUIRE(ix); VEIG_REQUIRE(ry); VEIG_REQUIRE(iy);
   const Geom G = make_geom(dim, block_cnt, thread_cnt);
   const int64_t grid = (G.nmat + 255) / 256;
   hipLaunchKernelGGL(k_zgemv, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, hermitian, G, rM, iM,
@@ -544,8 +339,8 @@ int veig_zgemv(int hermitian, int64_t dim, int64_t block_cnt, int64_t thread_cnt
 int veig_backsubstitution(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rT,
                           const double *iT, double *rb, double *ib, void *stream) {
   if (dim < 0 || block_cnt < 0 || thread_cnt < 1) return fail(VEIG_ERR_INVALID, "invalid batch geometry");
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
-  REQUIRE(rT); REQUIRE(iT); REQUIRE(rb); REQUIRE(ib);
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_REQUIRE(rT); VEIG_REQUIRE(iT); VEIG_REQUIRE(rb); VEIG_REQUIRE(ib);
   const Geom G = make_geom(dim, block_cnt, thread_cnt);
   const int64_t grid = (G.nmat + 255) / 256;
   hipLaunchKernelGGL(k_backsub, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, G, rT, iT, rb, ib);
@@ -556,8 +351,8 @@ int veig_backsubstitution(int64_t dim, int64_t block_cnt, int64_t thread_cnt, co
 static int layout(int dir, int64_t dim, int64_t bc, int64_t tc, const double *in, double *out,
                   void *stream) {
   if (dim < 0 || bc < 0 || tc < 1) return fail(VEIG_ERR_INVALID, "invalid batch geometry");
-  if (empty(dim, bc, tc)) return VEIG_OK;
-  REQUIRE(in); REQUIRE(out);
+  if (empty_batch(dim, bc, tc)) return VEIG_OK;
+  VEIG_REQUIRE(in); VEIG_REQUIRE(out);
   const Geom G = make_geom(dim, bc, tc);
   const int64_t total = (dir <= 2) ? G.nmat * dim * dim : G.nmat * dim;
   const int64_t grid = (total + 255) / 256;
@@ -585,69 +380,69 @@ int veig_rep_vector_soa(int64_t dim, int64_t bc, int64_t tc, const double *v, do
 // ---------------------------------------------------------------------------
 int veig_host_eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA,
                           double *rE, double *iE, double *rV, double *iV) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sv = (size_t)dim * block_cnt * thread_cnt, sm = sv * dim;
   DevBuf dra, dia, dre, die, drv, div;
-  TRY(dev_alloc(dra, sm)); TRY(dev_alloc(dia, sm));
-  TRY(dev_alloc(dre, sv)); TRY(dev_alloc(die, sv));
-  TRY(dev_alloc(drv, sm)); TRY(dev_alloc(div, sm));
-  TRY(h2d(dra, rA, sm)); TRY(h2d(dia, iA, sm));
-  TRY(veig_zgeev(dim, block_cnt, thread_cnt, dra.p, dia.p, dre.p, die.p, drv.p, div.p, nullptr, 30,
+  VEIG_TRY(dev_alloc(dra, sm)); VEIG_TRY(dev_alloc(dia, sm));
+  VEIG_TRY(dev_alloc(dre, sv)); VEIG_TRY(dev_alloc(die, sv));
+  VEIG_TRY(dev_alloc(drv, sm)); VEIG_TRY(dev_alloc(div, sm));
+  VEIG_TRY(h2d(dra, rA, sm)); VEIG_TRY(h2d(dia, iA, sm));
+  VEIG_TRY(veig_zgeev(dim, block_cnt, thread_cnt, dra.p, dia.p, dre.p, die.p, drv.p, div.p, nullptr, 30,
                  VEIG_WRITE_SCHUR, nullptr));
   VEIG_HIP(hipDeviceSynchronize());
-  TRY(d2h(rE, dre, sv)); TRY(d2h(iE, die, sv));
-  TRY(d2h(rV, drv, sm)); TRY(d2h(iV, div, sm));
-  TRY(d2h(rA, dra, sm)); TRY(d2h(iA, dia, sm));
+  VEIG_TRY(d2h(rE, dre, sv)); VEIG_TRY(d2h(iE, die, sv));
+  VEIG_TRY(d2h(rV, drv, sm)); VEIG_TRY(d2h(iV, div, sm));
+  VEIG_TRY(d2h(rA, dra, sm)); VEIG_TRY(d2h(iA, dia, sm));
   return VEIG_OK;
 }
 
 int veig_host_zgehrd(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rtau,
                      double *itau) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sv = (size_t)dim * block_cnt * thread_cnt, sm = sv * dim;
   DevBuf a, b, t, u;
-  TRY(dev_alloc(a, sm)); TRY(dev_alloc(b, sm)); TRY(dev_alloc(t, sv)); TRY(dev_alloc(u, sv));
-  TRY(h2d(a, rA, sm)); TRY(h2d(b, iA, sm)); TRY(h2d(t, rtau, sv)); TRY(h2d(u, itau, sv));
-  TRY(veig_zgehrd(dim, block_cnt, thread_cnt, a.p, b.p, t.p, u.p, nullptr));
+  VEIG_TRY(dev_alloc(a, sm)); VEIG_TRY(dev_alloc(b, sm)); VEIG_TRY(dev_alloc(t, sv)); VEIG_TRY(dev_alloc(u, sv));
+  VEIG_TRY(h2d(a, rA, sm)); VEIG_TRY(h2d(b, iA, sm)); VEIG_TRY(h2d(t, rtau, sv)); VEIG_TRY(h2d(u, itau, sv));
+  VEIG_TRY(veig_zgehrd(dim, block_cnt, thread_cnt, a.p, b.p, t.p, u.p, nullptr));
   VEIG_HIP(hipDeviceSynchronize());
-  TRY(d2h(rA, a, sm)); TRY(d2h(iA, b, sm)); TRY(d2h(rtau, t, sv)); TRY(d2h(itau, u, sv));
+  VEIG_TRY(d2h(rA, a, sm)); VEIG_TRY(d2h(iA, b, sm)); VEIG_TRY(d2h(rtau, t, sv)); VEIG_TRY(d2h(itau, u, sv));
   return VEIG_OK;
 }
 
 int veig_host_zunghr(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rtau,
                      double *itau) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sv = (size_t)dim * block_cnt * thread_cnt, sm = sv * dim;
   DevBuf a, b, t, u;
-  TRY(dev_alloc(a, sm)); TRY(dev_alloc(b, sm)); TRY(dev_alloc(t, sv)); TRY(dev_alloc(u, sv));
-  TRY(h2d(a, rA, sm)); TRY(h2d(b, iA, sm)); TRY(h2d(t, rtau, sv)); TRY(h2d(u, itau, sv));
-  TRY(veig_zunghr(dim, block_cnt, thread_cnt, a.p, b.p, t.p, u.p, nullptr));
+  VEIG_TRY(dev_alloc(a, sm)); VEIG_TRY(dev_alloc(b, sm)); VEIG_TRY(dev_alloc(t, sv)); VEIG_TRY(dev_alloc(u, sv));
+  VEIG_TRY(h2d(a, rA, sm)); VEIG_TRY(h2d(b, iA, sm)); VEIG_TRY(h2d(t, rtau, sv)); VEIG_TRY(h2d(u, itau, sv));
+  VEIG_TRY(veig_zunghr(dim, block_cnt, thread_cnt, a.p, b.p, t.p, u.p, nullptr));
   VEIG_HIP(hipDeviceSynchronize());
-  TRY(d2h(rA, a, sm)); TRY(d2h(iA, b, sm));
+  VEIG_TRY(d2h(rA, a, sm)); VEIG_TRY(d2h(iA, b, sm));
   return VEIG_OK;
 }
 
 int veig_host_hessenberg(int dim, int block_cnt, int thread_cnt, double *rA, double *iA,
                          double *rQ, double *iQ) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sm = (size_t)dim * dim * block_cnt * thread_cnt;
   DevBuf a, b, q, p;
-  TRY(dev_alloc(a, sm)); TRY(dev_alloc(b, sm)); TRY(dev_alloc(q, sm)); TRY(dev_alloc(p, sm));
-  TRY(h2d(a, rA, sm)); TRY(h2d(b, iA, sm));
-  TRY(veig_hessenberg(dim, block_cnt, thread_cnt, a.p, b.p, q.p, p.p, nullptr));
+  VEIG_TRY(dev_alloc(a, sm)); VEIG_TRY(dev_alloc(b, sm)); VEIG_TRY(dev_alloc(q, sm)); VEIG_TRY(dev_alloc(p, sm));
+  VEIG_TRY(h2d(a, rA, sm)); VEIG_TRY(h2d(b, iA, sm));
+  VEIG_TRY(veig_hessenberg(dim, block_cnt, thread_cnt, a.p, b.p, q.p, p.p, nullptr));
   VEIG_HIP(hipDeviceSynchronize());
-  TRY(d2h(rA, a, sm)); TRY(d2h(iA, b, sm)); TRY(d2h(rQ, q, sm)); TRY(d2h(iQ, p, sm));
+  VEIG_TRY(d2h(rA, a, sm)); VEIG_TRY(d2h(iA, b, sm)); VEIG_TRY(d2h(rQ, q, sm)); VEIG_TRY(d2h(iQ, p, sm));
   return VEIG_OK;
 }
 
 int veig_host_zlahqr_new(int dim, int block_cnt, int thread_cnt, double *rE, double *iE,
                          double *rM, double *iM, double *rS, double *iS) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sv = (size_t)dim * block_cnt * thread_cnt, sm = sv * dim;
   // S := 0 with the diagonal written through swapped planes (utils.cpp:167-169):
   // the imaginary plane gets 1.0 and the real plane 0.0, so S = i*I.
@@ -657,57 +452,57 @@ int veig_host_zlahqr_new(int dim, int block_cnt, int thread_cnt, double *rE, dou
     for (int64_t t = 0; t < thread_cnt; ++t)
       for (int64_t j = 0; j < dim; ++j) iS[b * thread_cnt * dim * dim + (j + j * dim) * thread_cnt + t] = 1.0;
   DevBuf e, f, a, b, s, u;
-  TRY(dev_alloc(e, sv)); TRY(dev_alloc(f, sv)); TRY(dev_alloc(a, sm)); TRY(dev_alloc(b, sm));
-  TRY(dev_alloc(s, sm)); TRY(dev_alloc(u, sm));
-  TRY(h2d(a, rM, sm)); TRY(h2d(b, iM, sm)); TRY(h2d(s, rS, sm)); TRY(h2d(u, iS, sm));
-  TRY(veig_zlahqr(dim, block_cnt, thread_cnt, e.p, f.p, a.p, b.p, s.p, u.p, nullptr, 30, nullptr));
+  VEIG_TRY(dev_alloc(e, sv)); VEIG_TRY(dev_alloc(f, sv)); VEIG_TRY(dev_alloc(a, sm)); VEIG_TRY(dev_alloc(b, sm));
+  VEIG_TRY(dev_alloc(s, sm)); VEIG_TRY(dev_alloc(u, sm));
+  VEIG_TRY(h2d(a, rM, sm)); VEIG_TRY(h2d(b, iM, sm)); VEIG_TRY(h2d(s, rS, sm)); VEIG_TRY(h2d(u, iS, sm));
+  VEIG_TRY(veig_zlahqr(dim, block_cnt, thread_cnt, e.p, f.p, a.p, b.p, s.p, u.p, nullptr, 30, nullptr));
   VEIG_HIP(hipDeviceSynchronize());
-  TRY(d2h(rE, e, sv)); TRY(d2h(iE, f, sv)); TRY(d2h(rM, a, sm)); TRY(d2h(iM, b, sm));
-  TRY(d2h(rS, s, sm)); TRY(d2h(iS, u, sm));
+  VEIG_TRY(d2h(rE, e, sv)); VEIG_TRY(d2h(iE, f, sv)); VEIG_TRY(d2h(rM, a, sm)); VEIG_TRY(d2h(iM, b, sm));
+  VEIG_TRY(d2h(rS, s, sm)); VEIG_TRY(d2h(iS, u, sm));
   return VEIG_OK;
 }
 
 int veig_host_ztrevc(int dim, int block_cnt, int thread_cnt, double *rT, double *iT, double *rR,
                      double *iR) {
-  TRY(check_geom(dim, block_cnt, thread_cnt));
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sm = (size_t)dim * dim * block_cnt * thread_cnt;
   DevBuf a, b, c, d;
-  TRY(dev_alloc(a, sm)); TRY(dev_alloc(b, sm)); TRY(dev_alloc(c, sm)); TRY(dev_alloc(d, sm));
-  TRY(h2d(a, rT, sm)); TRY(h2d(b, iT, sm)); TRY(h2d(c, rR, sm)); TRY(h2d(d, iR, sm));
-  TRY(veig_ztrevc(dim, block_cnt, thread_cnt, a.p, b.p, c.p, d.p, nullptr));
+  VEIG_TRY(dev_alloc(a, sm)); VEIG_TRY(dev_alloc(b, sm)); VEIG_TRY(dev_alloc(c, sm)); VEIG_TRY(dev_alloc(d, sm));
+  VEIG_TRY(h2d(a, rT, sm)); VEIG_TRY(h2d(b, iT, sm)); VEIG_TRY(h2d(c, rR, sm)); VEIG_TRY(h2d(d, iR, sm));
+  VEIG_TRY(veig_ztrevc(dim, block_cnt, thread_cnt, a.p, b.p, c.p, d.p, nullptr));
   VEIG_HIP(hipDeviceSynchronize());
-  TRY(d2h(rR, c, sm)); TRY(d2h(iR, d, sm));
+  VEIG_TRY(d2h(rR, c, sm)); VEIG_TRY(d2h(iR, d, sm));
   return VEIG_OK;
 }
 
 int veig_host_zgemv(int hermitian, int dim, int block_cnt, int thread_cnt, double *rM, double *iM,
                     double *rx, double *ix, double *ry, double *iy, double rbeta, double ibeta) {
   if (dim < 0 || block_cnt < 0 || thread_cnt < 1) return fail(VEIG_ERR_INVALID, "invalid batch geometry");
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sv = (size_t)dim * block_cnt * thread_cnt, sm = sv * dim;
   DevBuf a, b, x, y, u, v;
-  TRY(dev_alloc(a, sm)); TRY(dev_alloc(b, sm)); TRY(dev_alloc(x, sv)); TRY(dev_alloc(y, sv));
-  TRY(dev_alloc(u, sv)); TRY(dev_alloc(v, sv));
-  TRY(h2d(a, rM, sm)); TRY(h2d(b, iM, sm)); TRY(h2d(x, rx, sv)); TRY(h2d(y, ix, sv));
-  TRY(h2d(u, ry, sv)); TRY(h2d(v, iy, sv));
-  TRY(veig_zgemv(hermitian, dim, block_cnt, thread_cnt, a.p, b.p, x.p, y.p, u.p, v.p, rbeta, ibeta, nullptr));
+  VEIG_TRY(dev_alloc(a, sm)); VEIG_TRY(dev_alloc(b, sm)); VEIG_TRY(dev_alloc(x, sv)); VEIG_TRY(dev_alloc(y, sv));
+  VEIG_TRY(dev_alloc(u, sv)); VEIG_TRY(dev_alloc(v, sv));
+  VEIG_TRY(h2d(a, rM, sm)); VEIG_TRY(h2d(b, iM, sm)); VEIG_TRY(h2d(x, rx, sv)); VEIG_TRY(h2d(y, ix, sv));
+  VEIG_TRY(h2d(u, ry, sv)); VEIG_TRY(h2d(v, iy, sv));
+  VEIG_TRY(veig_zgemv(hermitian, dim, block_cnt, thread_cnt, a.p, b.p, x.p, y.p, u.p, v.p, rbeta, ibeta, nullptr));
   VEIG_HIP(hipDeviceSynchronize());
-  TRY(d2h(ry, u, sv)); TRY(d2h(iy, v, sv));
+  VEIG_TRY(d2h(ry, u, sv)); VEIG_TRY(d2h(iy, v, sv));
   return VEIG_OK;
 }
 
 int veig_host_backsubstitution(int dim, int block_cnt, int thread_cnt, double *rT, double *iT,
                                double *rb, double *ib) {
   if (dim < 0 || block_cnt < 0 || thread_cnt < 1) return fail(VEIG_ERR_INVALID, "invalid batch geometry");
-  if (empty(dim, block_cnt, thread_cnt)) return VEIG_OK;
+  if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sv = (size_t)dim * block_cnt * thread_cnt, sm = sv * dim;
   DevBuf a, b, x, y;
-  TRY(dev_alloc(a, sm)); TRY(dev_alloc(b, sm)); TRY(dev_alloc(x, sv)); TRY(dev_alloc(y, sv));
-  TRY(h2d(a, rT, sm)); TRY(h2d(b, iT, sm)); TRY(h2d(x, rb, sv)); TRY(h2d(y, ib, sv));
-  TRY(veig_backsubstitution(dim, block_cnt, thread_cnt, a.p, b.p, x.p, y.p, nullptr));
+  VEIG_TRY(dev_alloc(a, sm)); VEIG_TRY(dev_alloc(b, sm)); VEIG_TRY(dev_alloc(x, sv)); VEIG_TRY(dev_alloc(y, sv));
+  VEIG_TRY(h2d(a, rT, sm)); VEIG_TRY(h2d(b, iT, sm)); VEIG_TRY(h2d(x, rb, sv)); VEIG_TRY(h2d(y, ib, sv));
+  VEIG_TRY(veig_backsubstitution(dim, block_cnt, thread_cnt, a.p, b.p, x.p, y.p, nullptr));
   VEIG_HIP(hipDeviceSynchronize());
-  TRY(d2h(rb, x, sv)); TRY(d2h(ib, y, sv));
+  VEIG_TRY(d2h(rb, x, sv)); VEIG_TRY(d2h(ib, y, sv));
   return VEIG_OK;
 }
 

@@ -1,0 +1,260 @@
+// Shared host/device plumbing of the gfx950 eigensolver translation units:
+// batch geometry of the tiled split-plane layout, the per-wavefront LDS
+// tiles, row loads/stores, error reporting and the fused kernel template.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "veig_device.hpp"
+#include "../../include/veig.h"
+
+namespace veig {
+
+// last error message of the calling thread (veig_last_error)
+std::string &last_error();
+
+
+
+inline int fail(int status, const std::string &msg) {
+  last_error() = msg;
+  return status;
+}
+
+inline int hip_fail(hipError_t e, const char *what) {
+  return fail(VEIG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define VEIG_HIP(call)                                  \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+  } while (0)
+
+// Batch geometry shared by all kernels.
+struct Geom {
+  int64_t nmat;  // block_cnt * thread_cnt
+  int64_t T;     // thread_cnt (tile)
+  int n;         // order
+  __device__ __forceinline__ int64_t mat_base(int64_t m) const {
+    const int64_t b = m / T, t = m - b * T;
+    return b * T * (int64_t)n * n + t;
+  }
+  __device__ __forceinline__ int64_t vec_base(int64_t m, int mult = 1) const {
+    const int64_t b = m / T, t = m - b * T;
+    return (int64_t)mult * b * T * n + t;
+  }
+  __device__ __forceinline__ int64_t el(int r, int c) const { return ((int64_t)r + (int64_t)c * n) * T; }
+  __device__ __forceinline__ int64_t ve(int r) const { return (int64_t)r * T; }
+};
+
+inline int check_geom(int64_t dim, int64_t block_cnt, int64_t thread_cnt) {
+  if (dim < 0 || block_cnt < 0 || thread_cnt < 1)
+    return fail(VEIG_ERR_INVALID, "invalid batch geometry (dim < 0, block_cnt < 0 or thread_cnt < 1)");
+  if (dim > VEIG_MAX_DIM)
+    return fail(VEIG_ERR_UNSUPPORTED, "order " + std::to_string(dim) + " exceeds VEIG_MAX_DIM (" +
+                                          std::to_string(VEIG_MAX_DIM) + ")");
+  if (block_cnt > 0 && thread_cnt > INT64_MAX / block_cnt) return fail(VEIG_ERR_INVALID, "batch size overflows");
+  return VEIG_OK;
+}
+
+inline int nb_for(int64_t n) { return n <= 8 ? 8 : (n <= 16 ? 16 : 32); }
+
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Adjacent matrices share cache lines in the tiled layout, so remap the
+// launch order so that each XCD walks one contiguous range of matrices.
+__device__ __forceinline__ int64_t xcd_block() {
+  constexpr int kXcd = 8;
+  const int64_t g = gridDim.x, b = blockIdx.x;
+  const int64_t g8 = g / kXcd * kXcd;
+  if (b >= g8) return b;
+  return (b % kXcd) * (g8 / kXcd) + b / kXcd;
+}
+
+// --------------------------------------------------------------------------
+// common kernel prologue: lane group setup and LDS tile
+// --------------------------------------------------------------------------
+template <typename R, int NB>
+struct Tiles {
+  using L2 = typename lds2<R>::type;
+  static constexpr int MPW = 64 / NB;
+  static constexpr int LD = NB + 1;
+  L2 H[MPW * NB * LD];
+  L2 v[MPW * NB];
+};
+
+template <typename R, int NB>
+__device__ __forceinline__ Group<R, NB> make_group(Tiles<R, NB> &sm, int n_of_matrix) {
+  Group<R, NB> g;
+  const int lane = threadIdx.x;
+  g.mi = lane / NB;
+  g.r = lane % NB;
+  g.n = n_of_matrix;
+  g.H = sm.H + g.mi * NB * Group<R, NB>::LD;
+  g.vsh = sm.v + g.mi * NB;
+  // zero the whole tile: entries outside the matrix (and below the
+  // subdiagonal when only a Hessenberg part is loaded) read as exact zeros
+#pragma unroll
+  for (int c = 0; c < NB; ++c) st<R>(g.at(g.r, c), mk<R>(0, 0));
+  st<R>(g.vsh + g.r, mk<R>(0, 0));
+  return g;
+}
+
+// lane r loads row r of a full matrix into the tile
+template <typename R, int NB>
+__device__ __forceinline__ void load_rows(const Group<R, NB> &g, const Geom &G, int64_t base,
+                                          const double *re, const double *im, int lower_band) {
+  if (g.r < g.n) {
+    for (int c = 0; c < g.n; ++c) {
+      if (g.r <= c + lower_band) {
+        const int64_t idx = base + G.el(g.r, c);
+        st<R>(g.at(g.r, c), mk<R>((R)re[idx], (R)im[idx]));
+      }
+    }
+  }
+}
+
+// lane r loads row r of a matrix into registers
+template <typename R, int NB>
+__device__ __forceinline__ void load_row_regs(const Group<R, NB> &g, const Geom &G, int64_t base,
+                                              const double *re, const double *im, cx<R> (&z)[NB]) {
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    z[c] = mk<R>(0, 0);
+    if (g.r < g.n && c < g.n) {
+      const int64_t idx = base + G.el(g.r, c);
+      z[c] = mk<R>((R)re[idx], (R)im[idx]);
+    }
+  }
+}
+
+template <typename R, int NB>
+__device__ __forceinline__ void store_row_regs(const Group<R, NB> &g, const Geom &G, int64_t base,
+                                               double *re, double *im, const cx<R> (&z)[NB]) {
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    if (g.r < g.n && c < g.n) {
+      const int64_t idx = base + G.el(g.r, c);
+      re[idx] = (double)z[c].re;
+      im[idx] = (double)z[c].im;
+    }
+  }
+}
+
+// lane r stores row r of the tile (full matrix)
+template <typename R, int NB>
+__device__ __forceinline__ void store_rows(const Group<R, NB> &g, const Geom &G, int64_t base,
+                                           double *re, double *im) {
+  if (g.r < g.n) {
+    for (int c = 0; c < g.n; ++c) {
+      const int64_t idx = base + G.el(g.r, c);
+      const cx<R> v = ld<R>(g.at(g.r, c));
+      re[idx] = (double)v.re;
+      im[idx] = (double)v.im;
+    }
+  }
+}
+
+template <typename R, int NB>
+__device__ __forceinline__ void identity_rows(const Group<R, NB> &g, cx<R> (&z)[NB]) {
+#pragma unroll
+  for (int c = 0; c < NB; ++c) z[c] = mk<R>(c == g.r ? R(1) : R(0), 0);
+}
+
+// --------------------------------------------------------------------------
+// fused eigendecomposition: A -> (E, V)
+// --------------------------------------------------------------------------
+struct FusedArgs {
+  double *rA, *iA;
+  double *rE, *iE, *rV, *iV;
+  int32_t *info;
+  Geom G;
+  int itmax;
+  uint32_t flags;
+  const int32_t *mask;  // if set, only matrices with mask[m] != 0 are solved
+  float *rW, *iW;       // if set (WANT_W), W = V^{-1} in complex64 planes
+};
+
+template <typename R, int NB, bool WANT_W>
+__global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
+  __shared__ Tiles<R, NB> sm;
+  constexpr int MPW = Tiles<R, NB>::MPW;
+  const int64_t m = xcd_block() * MPW + threadIdx.x / NB;
+  bool valid = m < a.G.nmat;
+  if (valid && a.mask) valid = a.mask[m] != 0;
+  if (!__any(valid)) return;  // wave-uniform early exit (masked launches)
+  const int n = valid ? a.G.n : 0;
+  Group<R, NB> g = make_group<R, NB>(sm, n);
+  const int64_t base = valid ? a.G.mat_base(m) : 0;
+  const int64_t vbase = valid ? a.G.vec_base(m) : 0;
+  wave_sync();
+  load_rows<R, NB>(g, a.G, base, a.rA, a.iA, NB);
+  cx<R> z[NB];
+  identity_rows<R, NB>(g, z);
+  wave_sync();
+  const int nmax = a.G.n;  // every matrix of a launch has the same order
+  gehrd_q<R, NB>(g, z, nmax, true, false, nullptr);
+  const int nfail = lahqr<R, NB>(g, z, a.itmax, false);
+  if (a.flags & VEIG_WRITE_SCHUR) store_rows<R, NB>(g, a.G, base, a.rA, a.iA);
+  const cx<R> lam = trevc_solve<R, NB>(g, nmax);
+  if constexpr (WANT_W) {
+    cx<R> w[NB];
+    inverse_vectors<R, NB>(g, z, w, nmax);
+    // column r of W
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      if (g.r < g.n && k < g.n) {
+        const int64_t idx = base + a.G.el(k, g.r);
+        a.rW[idx] = (float)w[k].re;
+        a.iW[idx] = (float)w[k].im;
+      }
+    }
+  }
+  backtransform<R, NB>(g, z, nmax);
+  store_row_regs<R, NB>(g, a.G, base, a.rV, a.iV, z);
+  if (g.r < g.n) {
+    a.rE[vbase + a.G.ve(g.r)] = (double)lam.re;
+    a.iE[vbase + a.G.ve(g.r)] = (double)lam.im;
+  }
+  if (a.info && valid && g.r == 0) a.info[m] = nfail;
+}
+
+template <typename R, bool WANT_W = false>
+int launch_zgeev(const FusedArgs &a, hipStream_t s) {
+  const int nb = nb_for(a.G.n);
+  const int mpw = 64 / nb;
+  const int64_t grid = (a.G.nmat + mpw - 1) / mpw;
+  if (grid > INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");
+  switch (nb) {
+    case 8: hipLaunchKernelGGL((k_zgeev<R, 8, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_zgeev<R, 16, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_zgeev<R, 32, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
+  }
+  VEIG_HIP(hipGetLastError());
+  return VEIG_OK;
+}
+
+#define VEIG_TRY(x)                   \
+  do {                                \
+    int st_ = (x);                    \
+    if (st_ != VEIG_OK) return st_;   \
+  } while (0)
+
+#define VEIG_REQUIRE(p)                                                                     \
+  do {                                                                                      \
+    if (!(p)) return fail(VEIG_ERR_INVALID, std::string("NULL pointer argument: ") + #p);   \
+  } while (0)
+
+inline Geom make_geom(int64_t dim, int64_t bc, int64_t tc) {
+  Geom G;
+  G.nmat = bc * tc;
+  G.T = tc;
+  G.n = (int)dim;
+  return G;
+}
+
+inline bool empty_batch(int64_t dim, int64_t bc, int64_t tc) { return dim == 0 || bc == 0 || tc == 0; }
+
+}  // namespace veig

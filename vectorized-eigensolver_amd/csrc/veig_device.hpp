@@ -57,6 +57,43 @@ template <> struct limits<float> {
   static constexpr float tiny = FLT_MIN;
 };
 
+// --------------------------------------------------------------------------
+// reciprocal and square root from the hardware approximations (v_rcp_f64,
+// v_rsq_f64) refined by Newton's method (each step doubles the correct bits;
+// two steps reach full double precision from the ~2^-26 seeds), in place of
+// the IEEE division / sqrt expansions.  Arguments are finite and nonzero
+// where rcp/rsq are used (guarded by the callers); rsqrt-based sqrt maps 0 to 0.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double frcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ double frsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  // y <- y * (1.5 - 0.5 x y^2), written as y + y * (0.5 - 0.5 x y^2)
+  double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+__device__ __forceinline__ double fsqrt(double x) {
+  // s = x*rsq(x), then one Newton correction on s itself
+  if (!(x > 0.0)) return x == 0.0 ? 0.0 : __builtin_nan("");
+  const double y = frsq(x);
+  const double s = x * y;
+  const double d = fma(-s, s, x);
+  return fma(0.5 * y, d, s);
+}
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_sqrtf(x); }
+
 template <typename R> __device__ __forceinline__ cx<R> mk(R a, R b) { return cx<R>{a, b}; }
 template <typename R> __device__ __forceinline__ cx<R> add(cx<R> a, cx<R> b) { return {a.re + b.re, a.im + b.im}; }
 template <typename R> __device__ __forceinline__ cx<R> sub(cx<R> a, cx<R> b) { return {a.re - b.re, a.im - b.im}; }
@@ -77,7 +114,7 @@ template <typename R> __device__ __forceinline__ R cabs1(cx<R> a) { return fabs(
 template <typename R> __device__ __forceinline__ R abs2(cx<R> a) { return a.re * a.re + a.im * a.im; }
 // 1/a as conj(a)/|a|^2 (one real division)
 template <typename R> __device__ __forceinline__ cx<R> inv(cx<R> a) {
-  const R s = R(1) / abs2(a);
+  const R s = frcp(abs2(a));
   return {a.re * s, -a.im * s};
 }
 // a/b by Smith's method (ComplexDouble.hpp:319-353), robust to |b| range
@@ -92,8 +129,8 @@ template <typename R> __device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) {
 }
 // principal square root (ComplexDouble.hpp:451-488)
 template <typename R> __device__ __forceinline__ cx<R> csqrt(cx<R> a) {
-  const R zr = sqrt(R(0.5) * (sqrt(abs2(a)) + fabs(a.re)));
-  const R zi = (zr == R(0)) ? R(0) : fabs(a.im) / (R(2) * zr);
+  const R zr = fsqrt(R(0.5) * (fsqrt(abs2(a)) + fabs(a.re)));
+  const R zi = (zr == R(0)) ? R(0) : fabs(a.im) * (R(0.5) * frcp(zr));
   cx<R> o = (a.re < R(0)) ? mk(zi, zr) : mk(zr, zi);
   if (a.im < R(0)) o.im = -o.im;
   return o;
@@ -166,9 +203,9 @@ __device__ __forceinline__ void gehrd_q(const Group<R, NB>& g, cx<R> (&z)[NB], i
     cx<R> tau = mk<R>(0, 0), scal = mk<R>(0, 0);
     R beta = alpha.re;
     if (live && !(xnorm2 == R(0) && alpha.im == R(0))) {
-      const R nrm = sqrt(abs2(alpha) + xnorm2);
+      const R nrm = fsqrt(abs2(alpha) + xnorm2);
       beta = alpha.re >= R(0) ? -nrm : nrm;
-      const R ib = R(1) / beta;
+      const R ib = frcp(beta);
       tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
       scal = inv(mk(alpha.re - beta, alpha.im));
     }
@@ -232,10 +269,12 @@ __device__ __forceinline__ void larfg2(cx<R> a0, cx<R> a1, cx<R>& t1, cx<R>& v2,
     beta_out = a0;
     return;
   }
-  const R nrm = sqrt(abs2(a0) + abs2(a1));
+  const R nrm2 = abs2(a0) + abs2(a1);
+  const R rn = frsq(nrm2);                    // 1/norm
+  const R nrm = nrm2 * rn;
   const R beta = a0.re >= R(0) ? nrm : -nrm;  // copysign(norm, Re a0)
   const cx<R> zz = mk(a0.re + beta, a0.im);
-  const R ib = R(1) / beta;
+  const R ib = a0.re >= R(0) ? rn : -rn;      // 1/beta
   t1 = scale(zz, ib);
   v2 = mul(a1, inv(zz));
   beta_out = mk<R>(-beta, 0);
@@ -266,13 +305,12 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
       cx<R> sc = mk<R>(1, 0);
       if (doit) {
         const R d1 = cabs1(h);
-        cx<R> w = scale(h, R(1) / d1);
-        const R d2 = sqrt(abs2(w));
-        sc = scale(conj(w), R(1) / d2);  // conj(h/|h|)
+        cx<R> w = scale(h, frcp(d1));
+        sc = scale(conj(w), frsq(abs2(w)));  // conj(h/|h|)
       }
       wave_sync();
       if (doit) {
-        if (r == i) st<R>(g.at(i, i - 1), mk<R>(sqrt(abs2(h)), 0));
+        if (r == i) st<R>(g.at(i, i - 1), mk<R>(fsqrt(abs2(h)), 0));
         if (r >= i && r < n) st<R>(g.at(i, r), mul(ld<R>(g.at(i, r)), sc));
         if (r <= i + 1 && r < n) st<R>(g.at(r, i), mulc(ld<R>(g.at(r, i)), sc));
         z[i] = mulc(z[i], sc);
@@ -362,7 +400,7 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
           cx<R> x = scale(sub(ld<R>(g.at(I - 1, I - 1)), t), R(0.5));
           const R sx = cabs1(x);
           s = fmax(s, sx);
-          const R is = R(1) / s;
+          const R is = frcp(s);
           const cx<R> xs = scale(x, is), us = scale(u, is);
           cx<R> y = scale(csqrt(add(mul(xs, xs), mul(us, us))), s);
           if (sx > R(0)) {
@@ -374,7 +412,7 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
       const cx<R> h11s = sub(ld<R>(g.at(L, L)), t);
       const R h21 = ld<R>(g.at(L + 1, L)).re;
       const R s = cabs1(h11s) + fabs(h21);
-      const R is = R(1) / s;
+      const R is = frcp(s);
       a0 = scale(h11s, is);
       a1 = mk<R>(h21 * is, 0);
     }
@@ -447,8 +485,8 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
     if (ready) {
       const cx<R> h = ld<R>(g.at(I, I - 1));
       if (h.im != R(0)) {
-        const R d = sqrt(abs2(h));
-        const R id = R(1) / d;
+        const R id = frsq(abs2(h));
+        const R d = abs2(h) * id;
         const cx<R> zz = mk(h.re * id, -h.im * id);  // conj(h/|h|)
         if (r == I) st<R>(g.at(I, I - 1), mk<R>(d, 0));
         if (r > I && r < n) st<R>(g.at(I, r), mul(ld<R>(g.at(I, r)), zz));
@@ -465,22 +503,20 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
 }
 
 // --------------------------------------------------------------------------
-// Right eigenvectors of the upper-triangular T in LDS, back-transformed by the
-// Schur vectors (ztrevc + backsubstitution + zgemv, Toolbox.hpp:1133-1313,
-// 31-136, 1033-1131).  Lane k solves (T(0:k-1,0:k-1) - T(k,k) I) x = -T(0:k-1,k)
-// with the reference's small-pivot clamp; X (unit upper triangular) is written
-// over the strict upper triangle of T; then row r of V = Z*X is formed in
-// registers (column k of V = Z*[x; 1; 0], the reference's scaling).
-// On exit z[] holds row r of V and lam the eigenvalue T(r,r).
+// Right eigenvectors of the upper-triangular T in LDS (ztrevc +
+// backsubstitution, Toolbox.hpp:1133-1313, 31-136).  Lane k solves
+// (T(0:k-1,0:k-1) - T(k,k) I) x = -T(0:k-1,k) with the reference's
+// small-pivot clamp (Toolbox.hpp:1235-1258); the unit upper triangular X
+// (column k = [x; 1; 0]) is written over the strict upper triangle of T.
+// Returns lam = T(r,r) in lane r.
 // --------------------------------------------------------------------------
 template <typename R, int NB>
-__device__ __forceinline__ void trevc_backtransform(const Group<R, NB>& g, cx<R> (&z)[NB],
-                                                    cx<R>& lam, int nmax) {
+__device__ __forceinline__ cx<R> trevc_solve(const Group<R, NB>& g, int nmax) {
   const int r = g.r, n = g.n;
   const R ulp = limits<R>::eps;
   const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
   const bool in = r < n;
-  lam = in ? ld<R>(g.at(r, r)) : mk<R>(0, 0);
+  const cx<R> lam = in ? ld<R>(g.at(r, r)) : mk<R>(0, 0);
   const R smin = fmax(ulp * cabs1(lam), smlnum);
   cx<R> x[NB];
 #pragma unroll
@@ -509,7 +545,58 @@ __device__ __forceinline__ void trevc_backtransform(const Group<R, NB>& g, cx<R>
   for (int j = 0; j < NB; ++j)
     if (in && j < r) st<R>(g.at(j, r), x[j]);
   wave_sync();
-  // V(r, k) = z[k] + sum_{j<k} z[j] X(j,k); in place from the last column down
+  return lam;
+}
+
+// --------------------------------------------------------------------------
+// Approximate inverse of the eigenvector matrix, W = X^{-1} Z^H, from the X
+// left in LDS by trevc_solve and the Schur-vector row z[] of each lane.
+// Lane k forms column k of Y = X^{-1} by back substitution and stores its
+// strict upper part transposed into the (unused) strict lower triangle of the
+// tile; then lane r forms column r of W, returned in w[] (w[k] = W(k, r)).
+// --------------------------------------------------------------------------
+template <typename R, int NB>
+__device__ __forceinline__ void inverse_vectors(const Group<R, NB>& g, const cx<R> (&z)[NB],
+                                                cx<R> (&w)[NB], int nmax) {
+  const int r = g.r, n = g.n;
+  const bool in = r < n;
+  cx<R> y[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) y[j] = mk<R>(j == r ? R(1) : R(0), 0);
+  // y_j = -sum_{i=j+1..k} X(j,i) y_i, j = k-1 .. 0
+#pragma unroll
+  for (int j = NB - 2; j >= 0; --j) {
+    if (j + 1 < nmax) {
+      cx<R> acc = mk<R>(0, 0);
+#pragma unroll
+      for (int i = j + 1; i < NB; ++i) acc = add(acc, mul(ld<R>(g.at(j, i)), y[i]));
+      if (in && j < r) y[j] = mk(-acc.re, -acc.im);
+    }
+  }
+  // Y(j,k) (j < k) -> tile position (k, j)
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    if (in && j < r) st<R>(g.at(r, j), y[j]);
+  wave_sync();
+  // W(k, r) = conj(z[k]) + sum_{j>k} Y(k,j) conj(z[j])
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    cx<R> acc = conj(z[k]);
+    if (k < nmax) {
+#pragma unroll
+      for (int j = k + 1; j < NB; ++j) acc = add(acc, mulc(ld<R>(g.at(j, k)), z[j]));
+    }
+    w[k] = acc;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Back-transform (the zgemv of ztrevc, Toolbox.hpp:1272-1275): row r of
+// V = Z * X, in place in z[] (column k of V = Z [x; 1; 0], the reference's
+// scaling, not normalised).
+// --------------------------------------------------------------------------
+template <typename R, int NB>
+__device__ __forceinline__ void backtransform(const Group<R, NB>& g, cx<R> (&z)[NB], int nmax) {
 #pragma unroll
   for (int k = NB - 1; k >= 1; --k) {
     if (k < nmax) {
@@ -519,6 +606,13 @@ __device__ __forceinline__ void trevc_backtransform(const Group<R, NB>& g, cx<R>
       z[k] = acc;
     }
   }
+}
+
+template <typename R, int NB>
+__device__ __forceinline__ void trevc_backtransform(const Group<R, NB>& g, cx<R> (&z)[NB],
+                                                    cx<R>& lam, int nmax) {
+  lam = trevc_solve<R, NB>(g, nmax);
+  backtransform<R, NB>(g, z, nmax);
 }
 
 }  // namespace veig

@@ -44,7 +44,7 @@ _SIGS = {
     "veig_zgeev": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32,
                     ctypes.c_uint32, _vp], _i),
     "veig_zgeev_fp32": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp], _i),
-    "veig_zgeev_mixed": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+    "veig_zgeev_mixed": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                           ctypes.c_double, ctypes.c_int32, _vp, ctypes.c_size_t, _vp], _i),
     "veig_mixed_workspace_size": ([_i64, _i64, _i64], ctypes.c_size_t),
     "veig_zgehrd": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp], _i),
@@ -166,7 +166,8 @@ def mixed_workspace(dim: int, block_cnt: int, thread_cnt: int, device="cuda"):
 
 
 def zgeev_mixed(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, info=None,
-                berr=None, tol: float = 1e-12, itmax: int = 30, workspace=None, stream=None):
+                berr=None, fallback=None, tol: float = 1e-12, itmax: int = 30, workspace=None,
+                stream=None):
     """Mixed precision: complex64 reduction/QR/eigenvectors, then FP64
     residual-driven refinement of every eigenpair to backward error <= tol,
     with an FP64 re-solve of the matrices that do not get there.
@@ -184,8 +185,8 @@ def zgeev_mixed(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, inf
     if workspace is None:
         workspace = mixed_workspace(dim, block_cnt, thread_cnt, dev)
     st = lib().veig_zgeev_mixed(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
-                                _ptr(rV), _ptr(iV), _ptr(info), _ptr(berr), tol, itmax,
-                                _ptr(workspace), workspace.numel(), _stream(stream))
+                                _ptr(rV), _ptr(iV), _ptr(info), _ptr(fallback), _ptr(berr), tol,
+                                itmax, _ptr(workspace), workspace.numel(), _stream(stream))
     check(st, "veig_zgeev_mixed")
     return rE, iE, rV, iV
 
