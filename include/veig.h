@@ -94,6 +94,37 @@ int veig_zgeev_mixed(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const d
                      void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Batches of different orders.  No counterpart in the reference (its batches
+ * are equal-size, utils.cpp:578); the per-matrix arithmetic is veig_zgeev's.
+ * Matrix m (order dims[m], 0 <= dims[m] <= 32) is column-major with leading
+ * dimension dims[m] at offset a_off[m] of the rA/iA planes; its eigenvectors
+ * go to the same offsets of rV/iV and its eigenvalues to e_off[m] of rE/iE.
+ * The matrices are sorted into lane-group widths NB = 8/16/32 on the device
+ * (nb_of_n[n], may be NULL for the smallest width >= n) and solved by one
+ * persistent kernel per width (grid_per_bin[3] workgroups, NULL = default).
+ * info[m] (may be NULL): as veig_zgeev, -1 for an order outside [0, 32].
+ * Enqueues only kernels (no allocation, copy or memset), so it can be
+ * captured into a graph; veig_varsize_graph_create does that.
+ * workspace: >= veig_varsize_workspace_size(nmat) bytes, 256-byte aligned.
+ * ------------------------------------------------------------------------- */
+typedef struct veig_graph veig_graph;
+size_t veig_varsize_workspace_size(int64_t nmat);
+int veig_zgeev_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off,
+                       const int64_t *e_off, const double *rA, const double *iA, double *rE,
+                       double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
+                       const int8_t *nb_of_n, const int32_t *grid_per_bin, void *workspace,
+                       size_t workspace_bytes, void *stream);
+/* Capture the same launch sequence into an instantiated hipGraph (arguments as
+ * above; pointers are baked in, array contents are read at each launch). */
+int veig_varsize_graph_create(int64_t nmat, const int32_t *dims, const int64_t *a_off,
+                              const int64_t *e_off, const double *rA, const double *iA,
+                              double *rE, double *iE, double *rV, double *iV, int32_t *info,
+                              int32_t itmax, const int8_t *nb_of_n, const int32_t *grid_per_bin,
+                              void *workspace, size_t workspace_bytes, veig_graph **out);
+int veig_graph_launch(veig_graph *graph, void *stream);
+int veig_graph_destroy(veig_graph *graph);
+
+/* ---------------------------------------------------------------------------
  * Stage entries, device pointers (the batch kernels of Toolbox-def.hpp:5-22).
  * ------------------------------------------------------------------------- */
 /* test_zgehrd (Toolbox.hpp:2150) / call_zgehrd (utils.cpp:330): A -> upper

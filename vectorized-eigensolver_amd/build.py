@@ -36,16 +36,41 @@ def hipcc() -> str:
 
 
 def build_lib(verbose: bool = False) -> Path:
-    srcs = [str(CSRC / "veig.hip"), str(CSRC / "veig_mixed.hip"), str(CSRC / "veig_compat.cpp")]
-    srcs = [s for s in srcs if Path(s).exists()]
+    """Compile each translation unit to an object in parallel, then link."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    srcs = [CSRC / "veig.hip", CSRC / "veig_mixed.hip", CSRC / "veig_varsize.hip",
+            CSRC / "veig_compat.cpp"]
+    srcs = [s for s in srcs if s.exists()]
+    objdir = REPO / "_build"
+    objdir.mkdir(exist_ok=True)
+    cc = hipcc()
+
+    def compile_one(src: Path) -> Path:
+        obj = objdir / (src.name + ".o")
+        cmd = [cc, *[f for f in HIPCC_FLAGS if f != "-shared"], "-c", "-I", str(REPO / "include"),
+               "-o", str(obj), str(src)]
+        if src.suffix == ".cpp":
+            cmd = [cc, "-O3", "-std=c++17", "-fPIC", "-c", "-I", str(REPO / "include"), "-o", str(obj),
+                   str(src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed on {src.name} ({res.returncode})")
+        return obj
+
+    with ThreadPoolExecutor(max_workers=min(4, len(srcs))) as ex:
+        objs = list(ex.map(compile_one, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(REPO / "include"), "-o", str(tmp), *srcs]
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode})")
+        raise RuntimeError(f"link failed ({res.returncode})")
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,288 @@
+// Batches of matrices of different orders (1 <= n <= 32) in one enqueue,
+// ready for capture into a hipGraph.
+//
+// Layout: matrix m is column-major with leading dimension dims[m], element
+// (0,0) at a_off[m] of the rA/iA planes; V uses the same offsets in rV/iV;
+// eigenvalue 0 of matrix m sits at e_off[m] of rE/iE (the reference has no
+// uneven-size layout; this is its one-matrix-per-tile case, thread_cnt = 1,
+// with explicit offsets).
+//
+// Launch structure (no host synchronisation, no allocation, no memset, so the
+// whole sequence can be captured and replayed):
+//   k_bin_init     clears the size histogram (a fill kernel, not a memset)
+//   k_bin_count    histogram of the orders
+//   k_bin_scan     orders grouped by lane-group width NB (8, 16 or 32, the
+//                  per-order choice of `nb_of_n`), ordered by n inside a group:
+//                  start offsets and the three bin ranges
+//   k_bin_scatter  permutation of matrix indices in that order
+//   k_zgeev_var<NB> one persistent kernel per bin: a fixed grid walks the bin's
+//                  range, read from device memory, wavefront by wavefront; the
+//                  matrices of a wavefront then have (nearly) the same order.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "veig_common.hpp"
+
+using namespace veig;
+
+namespace {
+
+constexpr int kMaxN = 32;
+constexpr int kBins = 3;  // NB = 8, 16, 32
+
+struct BinState {
+  int32_t hist[kMaxN + 1];    // count per order
+  int32_t cursor[kMaxN + 1];  // scatter cursor per order
+  int64_t start[kBins];       // bin range in the permutation
+  int64_t count[kBins];
+  int32_t nb_of_n[kMaxN + 1]; // lane-group width per order
+};
+
+struct VarArgs {
+  int64_t nmat;
+  const int32_t *dims;
+  const int64_t *a_off, *e_off;
+  const double *rA, *iA;
+  double *rE, *iE, *rV, *iV;
+  int32_t *info;
+  int32_t *perm;
+  BinState *bins;
+  int itmax;
+};
+
+__global__ void k_bin_init(BinState *b, const int8_t *nb_table) {
+  const int i = threadIdx.x;
+  if (i <= kMaxN) {
+    b->hist[i] = 0;
+    b->cursor[i] = 0;
+    int nb = nb_table ? nb_table[i] : 0;
+    if (nb != 8 && nb != 16 && nb != 32) nb = i <= 8 ? 8 : (i <= 16 ? 16 : 32);
+    if (nb < i) nb = i <= 8 ? 8 : (i <= 16 ? 16 : 32);  // a group must hold the matrix
+    b->nb_of_n[i] = nb;
+  }
+  if (i < kBins) {
+    b->start[i] = 0;
+    b->count[i] = 0;
+  }
+}
+
+__global__ void k_bin_count(VarArgs a) {
+  __shared__ int32_t h[kMaxN + 1];
+  if (threadIdx.x <= kMaxN) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < a.nmat;
+       m += (int64_t)gridDim.x * blockDim.x) {
+    const int n = a.dims[m];
+    if (n >= 1 && n <= kMaxN) {
+      atomicAdd(&h[n], 1);
+    } else if (a.info) {
+      a.info[m] = (n == 0) ? 0 : -1;  // n == 0: nothing to do; invalid order: -1
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x <= kMaxN && h[threadIdx.x]) atomicAdd(&a.bins->hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_bin_scan(BinState *b) {
+  if (threadIdx.x != 0) return;
+  int64_t pos = 0;
+  for (int bi = 0; bi < kBins; ++bi) {
+    const int nb = 8 << bi;
+    b->start[bi] = pos;
+    for (int n = 1; n <= kMaxN; ++n) {
+      if (b->nb_of_n[n] != nb) continue;
+      b->cursor[n] = (int32_t)pos;
+      pos += b->hist[n];
+    }
+    b->count[bi] = pos - b->start[bi];
+  }
+}
+
+__global__ void k_bin_scatter(VarArgs a) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < a.nmat;
+       m += (int64_t)gridDim.x * blockDim.x) {
+    const int n = a.dims[m];
+    if (n >= 1 && n <= kMaxN) a.perm[atomicAdd(&a.bins->cursor[n], 1)] = (int32_t)m;
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
+  using R = double;
+  __shared__ Tiles<R, NB> sm;
+  constexpr int MPW = Tiles<R, NB>::MPW;
+  const int64_t start = a.bins->start[bin], count = a.bins->count[bin];
+  const int64_t nwaves = (count + MPW - 1) / MPW;
+  for (int64_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
+    const int64_t idx = w * MPW + threadIdx.x / NB;
+    const bool valid = idx < count;
+    const int64_t m = valid ? a.perm[start + idx] : 0;
+    const int n = valid ? a.dims[m] : 0;
+    int nmax = 0;
+#pragma unroll
+    for (int q = 0; q < MPW; ++q) {
+      const int nq = __builtin_amdgcn_readlane(n, q * NB);
+      nmax = nq > nmax ? nq : nmax;
+    }
+    Group<R, NB> g = make_group<R, NB>(sm, n);
+    Geom G;
+    G.nmat = 0;
+    G.T = 1;
+    G.n = n;
+    const int64_t base = valid ? a.a_off[m] : 0;
+    const int64_t vbase = valid ? a.e_off[m] : 0;
+    wave_sync();
+    load_rows<R, NB>(g, G, base, a.rA, a.iA, NB);
+    cx<R> z[NB];
+    identity_rows<R, NB>(g, z);
+    wave_sync();
+    gehrd_q<R, NB>(g, z, nmax, true, false, nullptr);
+    const int nfail = lahqr<R, NB>(g, z, a.itmax, false);
+    const cx<R> lam = trevc_solve<R, NB>(g, nmax);
+    backtransform<R, NB>(g, z, nmax);
+    store_row_regs<R, NB>(g, G, base, a.rV, a.iV, z);
+    if (g.r < g.n) {
+      a.rE[vbase + g.r] = lam.re;
+      a.iE[vbase + g.r] = lam.im;
+    }
+    if (a.info && valid && g.r == 0) a.info[m] = nfail;
+    wave_sync();
+  }
+}
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+size_t varsize_ws_bytes(int64_t nmat) {
+  return align_up(sizeof(BinState)) + align_up((size_t)nmat * sizeof(int32_t));
+}
+
+int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, const int64_t *e_off,
+                    const double *rA, const double *iA, double *rE, double *iE, double *rV, double *iV,
+                    int32_t *info, int32_t itmax, const int8_t *nb_table, const int32_t *grid_per_bin,
+                    void *workspace, hipStream_t s) {
+  char *p = static_cast<char *>(workspace);
+  VarArgs a;
+  a.nmat = nmat;
+  a.dims = dims;
+  a.a_off = a_off;
+  a.e_off = e_off;
+  a.rA = rA;
+  a.iA = iA;
+  a.rE = rE;
+  a.iE = iE;
+  a.rV = rV;
+  a.iV = iV;
+  a.info = info;
+  a.bins = reinterpret_cast<BinState *>(p);
+  a.perm = reinterpret_cast<int32_t *>(p + align_up(sizeof(BinState)));
+  a.itmax = itmax > 0 ? itmax : 30;
+  int dev = 0, cus = 256;
+  VEIG_HIP(hipGetDevice(&dev));
+  VEIG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t cgrid64 = (nmat + 255) / 256;
+  const unsigned cgrid = (unsigned)(cgrid64 < 4 * (int64_t)cus ? (cgrid64 > 0 ? cgrid64 : 1) : 4 * cus);
+  hipLaunchKernelGGL(k_bin_init, dim3(1), dim3(64), 0, s, a.bins, nb_table);
+  hipLaunchKernelGGL(k_bin_count, dim3(cgrid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(cgrid), dim3(256), 0, s, a);
+  // persistent grids: a few resident waves per CU and bin (the three bins
+  // run back to back; every wave of a bin loops until the bin is drained)
+  const int defaults[kBins] = {8 * cus, 8 * cus, 4 * cus};
+  for (int b = 0; b < kBins; ++b) {
+    const int g = (grid_per_bin && grid_per_bin[b] > 0) ? grid_per_bin[b] : defaults[b];
+    switch (b) {
+      case 0: hipLaunchKernelGGL((k_zgeev_var<8>), dim3(g), dim3(64), 0, s, a, 0); break;
+      case 1: hipLaunchKernelGGL((k_zgeev_var<16>), dim3(g), dim3(64), 0, s, a, 1); break;
+      default: hipLaunchKernelGGL((k_zgeev_var<32>), dim3(g), dim3(64), 0, s, a, 2); break;
+    }
+  }
+  VEIG_HIP(hipGetLastError());
+  return VEIG_OK;
+}
+
+}  // namespace
+
+struct veig_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+size_t veig_varsize_workspace_size(int64_t nmat) {
+  if (nmat < 0) return 0;
+  return varsize_ws_bytes(nmat);
+}
+
+int veig_zgeev_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, const int64_t *e_off,
+                       const double *rA, const double *iA, double *rE, double *iE, double *rV,
+                       double *iV, int32_t *info, int32_t itmax, const int8_t *nb_of_n,
+                       const int32_t *grid_per_bin, void *workspace, size_t workspace_bytes,
+                       void *stream) {
+  if (nmat < 0) return fail(VEIG_ERR_INVALID, "nmat < 0");
+  if (nmat == 0) return VEIG_OK;
+  if (nmat > INT32_MAX) return fail(VEIG_ERR_INVALID, "nmat exceeds 2^31-1");
+  VEIG_REQUIRE(dims); VEIG_REQUIRE(a_off); VEIG_REQUIRE(e_off); VEIG_REQUIRE(rA); VEIG_REQUIRE(iA);
+  VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV); VEIG_REQUIRE(iV); VEIG_REQUIRE(workspace);
+  if (workspace_bytes < varsize_ws_bytes(nmat))
+    return fail(VEIG_ERR_INVALID, "workspace too small: need " + std::to_string(varsize_ws_bytes(nmat)) + " bytes");
+  if (reinterpret_cast<uintptr_t>(workspace) % kAlign != 0)
+    return fail(VEIG_ERR_INVALID, "workspace must be 256-byte aligned");
+  return enqueue_varsize(nmat, dims, a_off, e_off, rA, iA, rE, iE, rV, iV, info, itmax, nb_of_n,
+                         grid_per_bin, workspace, (hipStream_t)stream);
+}
+
+int veig_varsize_graph_create(int64_t nmat, const int32_t *dims, const int64_t *a_off,
+                              const int64_t *e_off, const double *rA, const double *iA, double *rE,
+                              double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
+                              const int8_t *nb_of_n, const int32_t *grid_per_bin, void *workspace,
+                              size_t workspace_bytes, veig_graph **out) {
+  VEIG_REQUIRE(out);
+  *out = nullptr;
+  hipStream_t cs;
+  VEIG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return hip_fail(e, "hipStreamBeginCapture");
+  }
+  const int st = veig_zgeev_varsize(nmat, dims, a_off, e_off, rA, iA, rE, iE, rV, iV, info, itmax,
+                                    nb_of_n, grid_per_bin, workspace, workspace_bytes, cs);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(cs, &graph);
+  (void)hipStreamDestroy(cs);
+  if (st != VEIG_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  veig_graph *g = new veig_graph;
+  g->graph = graph;
+  e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    delete g;
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  *out = g;
+  return VEIG_OK;
+}
+
+int veig_graph_launch(veig_graph *g, void *stream) {
+  VEIG_REQUIRE(g);
+  VEIG_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return VEIG_OK;
+}
+
+int veig_graph_destroy(veig_graph *g) {
+  if (!g) return VEIG_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return VEIG_OK;
+}
+
+}  // extern "C"

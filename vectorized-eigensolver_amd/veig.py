@@ -67,8 +67,14 @@ _SIGS = {
     "veig_host_ztrevc": ([_i, _i, _i, _D, _D, _D, _D], _i),
     "veig_host_zgemv": ([_i, _i, _i, _i, _D, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double], _i),
     "veig_host_backsubstitution": ([_i, _i, _i, _D, _D, _D, _D], _i),
-    "veig_zgeev_varsize": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                            ctypes.c_int32, _vp], _i),
+    "veig_varsize_workspace_size": ([_i64], ctypes.c_size_t),
+    "veig_zgeev_varsize": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32,
+                            _vp, _vp, _vp, ctypes.c_size_t, _vp], _i),
+    "veig_varsize_graph_create": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   ctypes.c_int32, _vp, _vp, _vp, ctypes.c_size_t,
+                                   ctypes.POINTER(ctypes.c_void_p)], _i),
+    "veig_graph_launch": ([_vp, _vp], _i),
+    "veig_graph_destroy": ([_vp], _i),
 }
 
 
@@ -118,7 +124,7 @@ def _ptr(t):
         raise VeigError("device entry points need GPU tensors")
     if not t.is_contiguous():
         raise VeigError("tensors must be contiguous")
-    if t.dtype not in (torch.float64, torch.int32, torch.uint8):
+    if t.dtype not in (torch.float64, torch.int32, torch.int64, torch.int8, torch.uint8):
         raise VeigError(f"unexpected dtype {t.dtype}")
     return ctypes.c_void_p(t.data_ptr())
 
@@ -223,3 +229,70 @@ def layout(kind: str, dim, bc, tc, src, dst, stream=None):
     name = {"soa": "veig_rearrange_matrices", "aos": "veig_arrange_matrices_aos",
             "repm": "veig_rep_matrix_soa", "repv": "veig_rep_vector_soa"}[kind]
     _stage(name, dim, bc, tc, src, dst, stream=stream)
+
+
+# --------------------------------------------------------------------------
+# batches of different orders
+# --------------------------------------------------------------------------
+class VarBatch:
+    """Device buffers for a batch of matrices of orders dims (1-D int tensor):
+    A/V planes in the per-matrix column-major layout, E planes, offsets,
+    info and workspace.  Fill rA/iA, then call run() or graph()."""
+
+    def __init__(self, dims, device="cuda"):
+        torch = _torch()
+        d = torch.as_tensor(dims, dtype=torch.int64)
+        self.nmat = int(d.numel())
+        sq = d * d
+        a_off = torch.zeros_like(d)
+        e_off = torch.zeros_like(d)
+        if self.nmat > 1:
+            a_off[1:] = torch.cumsum(sq, 0)[:-1]
+            e_off[1:] = torch.cumsum(d, 0)[:-1]
+        self.total_a = int(sq.sum())
+        self.total_e = int(d.sum())
+        dev = torch.device(device)
+        self.dims = d.to(torch.int32).to(dev)
+        self.a_off = a_off.to(dev)
+        self.e_off = e_off.to(dev)
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.rA = torch.zeros(self.total_a, **f64)
+        self.iA = torch.zeros(self.total_a, **f64)
+        self.rV = torch.zeros(self.total_a, **f64)
+        self.iV = torch.zeros(self.total_a, **f64)
+        self.rE = torch.zeros(self.total_e, **f64)
+        self.iE = torch.zeros(self.total_e, **f64)
+        self.info = torch.zeros(self.nmat, dtype=torch.int32, device=dev)
+        nbytes = lib().veig_varsize_workspace_size(self.nmat)
+        self.ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+        self.nb_table = None
+        self.grid = None
+
+    def _args(self, itmax):
+        return (self.nmat, _ptr(self.dims), _ptr(self.a_off), _ptr(self.e_off), _ptr(self.rA),
+                _ptr(self.iA), _ptr(self.rE), _ptr(self.iE), _ptr(self.rV), _ptr(self.iV),
+                _ptr(self.info), itmax, _ptr(self.nb_table), _ptr(self.grid), _ptr(self.ws),
+                self.ws.numel())
+
+    def run(self, itmax: int = 30, stream=None):
+        check(lib().veig_zgeev_varsize(*self._args(itmax), _stream(stream)), "veig_zgeev_varsize")
+
+    def graph(self, itmax: int = 30):
+        """An instantiated hipGraph of the whole sequence (binning + solves)."""
+        h = ctypes.c_void_p()
+        check(lib().veig_varsize_graph_create(*self._args(itmax), ctypes.byref(h)),
+              "veig_varsize_graph_create")
+        return Graph(h)
+
+
+class Graph:
+    def __init__(self, handle):
+        self.h = handle
+
+    def launch(self, stream=None):
+        check(lib().veig_graph_launch(self.h, _stream(stream)), "veig_graph_launch")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.veig_graph_destroy(self.h)
+            self.h = None
