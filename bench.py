@@ -1,16 +1,20 @@
 """Benchmark: batched complex eigendecomposition throughput on one MI355X.
 
-Workload (default): 1,048,576 complex128 16x16 matrices, real and imaginary
-parts iid N(0,1) from a fixed seed, in the tiled split-plane layout with tile
-64 (block_cnt 16384, thread_cnt 64); one step = one fused eigendecomposition
-(eigenvalues + right eigenvectors) of the whole batch, inputs resident in HBM.
+Default workload (--config c2, the headline): 1,048,576 complex128 16x16
+matrices, real and imaginary parts iid N(0,1) from a fixed seed, in the tiled
+split-plane layout with tile 64 (block_cnt 16384, thread_cnt 64).  One step =
+one eigendecomposition (eigenvalues + right eigenvectors) of the whole batch,
+inputs resident in HBM.
 
-Prints ONE JSON line.  `value` is the FP64 path's matrices/s; the same line
-carries the mixed-precision path's rate and backward error, the roofline of
-the dominant kernel (HIP events on the launch stream), and the reference
-OpenMP CPU path timed on a bounded sample of the same inputs.
+Other configurations (--config c1..c5) are described in
+vectorized-eigensolver_amd/workloads.py; c4 is the mixed-precision workload
+(its `value` is the mixed path's rate), c3 the batch of orders 4..32 run as
+one captured graph.
 
-    python bench.py [--steps K] [--warmup W] [--n 16] [--batch 1048576] [--tile 64]
+Prints ONE JSON line: `value` (matrices/s), the roofline of the dominant
+kernel timed with HIP events on its launch stream, the mixed-precision rate
+beside the FP64 one, and the reference's OpenMP CPU path timed on a bounded
+sample of the same inputs.
 """
 from __future__ import annotations
 
@@ -27,6 +31,7 @@ sys.path.insert(0, str(REPO / "tests"))
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
+METRIC = "eigendecomp matrices/s, n=16 c128: FP64 vs mixed-prec at backward err <=1e-12"
 
 
 def nominal_flops(n: int, sweeps_per_eig: float = 2.0) -> float:
@@ -45,15 +50,51 @@ def alg_bytes(n: int) -> int:
     return 32 * n * n + 16 * n
 
 
+def progress(msg):
+    """Phase boundaries on stderr (the JSON line stays the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, time.perf_counter() - t0
+
+
+def roofline(flops_per_launch, bytes_per_launch, kernel_ms, kernel, traffic_key):
+    tf = flops_per_launch / (kernel_ms * 1e-3) / 1e12
+    r = {"bound": "mfma", "achieved": round(tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+         "frac": round(tf / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": kernel,
+         "kernel_ms": round(kernel_ms, 4),
+         "alg_bytes_per_launch": bytes_per_launch,
+         "hbm_alg_GBs": round(bytes_per_launch / (kernel_ms * 1e-3) / 1e9, 1)}
+    p = REPO / "profiles" / "traffic.json"
+    if p.exists():
+        try:
+            r["traffic"] = json.loads(p.read_text()).get(traffic_key, {}).get("bytes_per_launch")
+        except Exception:
+            pass
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=16)
-    ap.add_argument("--batch", type=int, default=1 << 20)
-    ap.add_argument("--tile", type=int, default=64)
-    ap.add_argument("--seed", type=int, default=1002)
+    ap.add_argument("--batch", type=int, default=0, help="override the configuration's batch size")
+    ap.add_argument("--tile", type=int, default=0, help="thread_cnt of the layout (default: 64, c5: 1)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -68,102 +109,20 @@ def main():
               "(see the sharding note in DESIGN.md); use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import veig
+    import workloads as W
 
-    n, N, T = a.n, a.batch, a.tile
-    assert N % T == 0
-    bc = N // T
+    cfg = dict(W.CONFIGS[a.config])
+    if a.batch:
+        cfg["batch"] = a.batch
     dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(a.seed)
-    rA = torch.randn(N * n * n, dtype=torch.float64, device=dev, generator=g)
-    iA = torch.randn(N * n * n, dtype=torch.float64, device=dev, generator=g)
-    rE = torch.empty(N * n, dtype=torch.float64, device=dev)
-    iE = torch.empty_like(rE)
-    rV = torch.empty(N * n * n, dtype=torch.float64, device=dev)
-    iV = torch.empty_like(rV)
-    info = torch.empty(N, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
-
-    def step_fp64():
-        veig.zgeev(rA, iA, n, bc, T, out=(rE, iE, rV, iV), info=info, stream=stream)
-
-    for _ in range(a.warmup):
-        step_fp64()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(a.steps):
-        step_fp64()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = ev0.elapsed_time(ev1) / a.steps  # one launch per step
-    value = N / (ms_per_step * 1e-3)
-
-    # correctness of the timed outputs on a sample (backward error, info)
-    nfail = int((info != 0).sum())
-    S = min(N, 4096)
-    A_s = _gather(rA, iA, n, T, S)
-    E_s = _gather_vec(rE, iE, n, T, S)
-    V_s = _gather(rV, iV, n, T, S)
-    R = A_s @ V_s - V_s * E_s[:, None, :]
-    berr = (torch.linalg.norm(R, dim=(1, 2)) /
-            (torch.linalg.norm(A_s, dim=(1, 2)) * torch.linalg.norm(V_s, dim=(1, 2))))
-    max_berr = float(berr.max())
-
-    flops = nominal_flops(n)
-    achieved_tf = N * flops / (ms_per_step * 1e-3) / 1e12
-    roofline = {
-        "bound": "mfma", "achieved": round(achieved_tf, 3), "peak": PEAK_FP64_TFLOPS,
-        "unit": "TFLOP/s", "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4), "traffic": None,
-        "kernel": f"k_zgeev<double,{16 if n > 8 else 8}>",
-        "flops_per_matrix": flops,
-        "hbm_alg_GBs": round(N * alg_bytes(n) / (ms_per_step * 1e-3) / 1e9, 1),
-    }
-    tf = Path(REPO / "profiles" / f"traffic_zgeev_fp64_n{n}.json")
-    if tf.exists():
-        try:
-            roofline["traffic"] = json.loads(tf.read_text()).get("bytes_per_launch")
-        except Exception:
-            pass
-
-    mixed = None
-    if not a.no_mixed and hasattr(veig.lib(), "veig_zgeev_mixed"):
-        mixed = _bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, value)
-
-    cpu = None
-    if not a.no_cpu:
-        cpu = _cpu_baseline(rA, iA, n, T, N, a.cpu_seconds)
-
-    out = {
-        "metric": f"eigendecomp matrices/s, n={n} c128: FP64 vs mixed-prec at backward err <=1e-12",
-        "value": round(value, 1),
-        "unit": "matrices/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{N} complex128 {n}x{n} matrices, re/im iid N(0,1) seed {a.seed}, "
-                               f"tiled SoA tile {T} (block_cnt {bc}, thread_cnt {T}); "
-                               "eigenvalues + right eigenvectors",
-                   "batch": N, "n": n, "thread_cnt": T, "block_cnt": bc},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "mixed": mixed,
-        "check": {"sample": S, "max_backward_err": max_berr, "unconverged_matrices": nfail},
-        "wall_s_timed": round(wall, 3),
-    }
+    if a.config == "c3":
+        out = bench_varsize(a, cfg, veig, W, torch, dev, stream)
+    else:
+        out = bench_uniform(a, cfg, veig, W, torch, dev, stream)
     print(json.dumps(out), flush=True)
 
     if a.calibrate_copy:
@@ -177,30 +136,104 @@ def main():
               flush=True)
 
 
-def _gather(rX, iX, n, T, S):
+def bench_uniform(a, cfg, veig, W, torch, dev, stream):
+    n, N = cfg["n"], cfg["batch"]
+    T = min(a.tile or cfg.get("tile", 64), N)
+    assert N % T == 0
+    bc = N // T
+    progress(f"{a.config}: generating {N} inputs of order {n}")
+    if a.config == "c4":
+        rA, iA = W.c4_soa(N, n, T, cfg["seed"], dev)
+        workload = (f"{N} complex128 {n}x{n} matrices A = V diag(lam) V^-1, lam iid CN(0,1), cond(V) "
+                    f"log-uniform in [1e1, 1e6], seed {cfg['seed']}, tiled SoA tile {T}")
+    else:
+        rA, iA = W.gaussian_soa(N, n, T, cfg["seed"], dev)
+        workload = (f"{N} complex128 {n}x{n} matrices, re/im iid N(0,1) seed {cfg['seed']}, "
+                    f"tiled SoA tile {T} (block_cnt {bc}, thread_cnt {T})")
+    f64 = dict(dtype=torch.float64, device=dev)
+    rE, iE = torch.empty(N * n, **f64), torch.empty(N * n, **f64)
+    rV, iV = torch.empty(N * n * n, **f64), torch.empty(N * n * n, **f64)
+    info = torch.empty(N, dtype=torch.int32, device=dev)
+
+    def step_fp64():
+        veig.zgeev(rA, iA, n, bc, T, out=(rE, iE, rV, iV), info=info, stream=stream)
+
+    progress("timing the FP64 path")
+    ms64, wall64 = timed(step_fp64, a.steps, a.warmup, stream)
+    v64 = N / (ms64 * 1e-3)
+    chk64 = sample_check(rA, iA, rE, iE, rV, iV, n, T, N, info, torch)
+
+    mixed = None
+    if not a.no_mixed:
+        progress("timing the mixed-precision path")
+        mixed = bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, v64)
+
+    flops = N * nominal_flops(n)
+    nbytes = N * alg_bytes(n)
+    if a.config == "c4" and mixed is not None:
+        value, ms, dtype = mixed["value"], mixed["ms_per_step"], "c64+f64"
+        rf = roofline(flops, nbytes, mixed["fp32_kernel_ms"], f"k_zgeev<float,{nb(n)},W>", f"zgeev_fp32_n{n}")
+    else:
+        value, ms, dtype = v64, ms64, "f64"
+        rf = roofline(flops, nbytes, ms64, f"k_zgeev<double,{nb(n)}>", f"zgeev_fp64_n{n}")
+    rf["flops_per_matrix"] = nominal_flops(n)
+
+    if not a.no_cpu:
+        progress("timing the reference CPU path on a sample")
+    cpu = None if a.no_cpu else cpu_baseline_uniform(rA, iA, n, T, N, a.cpu_seconds)
+    return {
+        "metric": METRIC if a.config == "c2" else f"eigendecomp matrices/s, {a.config}: {workload}",
+        "value": round(value, 1), "unit": "matrices/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+        "config": {"workload": workload + "; eigenvalues + right eigenvectors", "name": a.config,
+                   "batch": N, "n": n, "thread_cnt": T, "block_cnt": bc},
+        "roofline": rf, "cpu_baseline": cpu,
+        "fp64": {"value": round(v64, 1), "ms_per_step": round(ms64, 4), **chk64},
+        "mixed": mixed, "wall_s_timed": round(wall64, 3),
+    }
+
+
+def nb(n):
+    return 8 if n <= 8 else (16 if n <= 16 else 32)
+
+
+def sample_check(rA, iA, rE, iE, rV, iV, n, T, N, info, torch):
+    """Backward error of the timed outputs on the first matrices; info."""
+    S = min(N, 4096)
+    A_s = gather(rA, iA, n, T, S)
+    E_s = gather_vec(rE, iE, n, T, S)
+    V_s = gather(rV, iV, n, T, S)
+    R = A_s @ V_s - V_s * E_s[:, None, :]
+    berr = (torch.linalg.norm(R, dim=(1, 2)) /
+            (torch.linalg.norm(A_s, dim=(1, 2)) * torch.linalg.norm(V_s, dim=(1, 2))))
+    return {"check_sample": S, "max_backward_err": float(berr.max()),
+            "unconverged_matrices": int((info != 0).sum())}
+
+
+def gather(rX, iX, n, T, S):
     """First S matrices of a tiled-SoA pair of planes as an (S,n,n) complex tensor."""
     import torch
-    bcs = S // T
+    bcs = max(1, S // T)
     X = torch.complex(rX[: bcs * T * n * n], iX[: bcs * T * n * n]).view(bcs, n, n, T)  # [b,c,r,t]
     return X.permute(0, 3, 2, 1).reshape(bcs * T, n, n).contiguous()
 
 
-def _gather_vec(rX, iX, n, T, S):
+def gather_vec(rX, iX, n, T, S):
     import torch
-    bcs = S // T
+    bcs = max(1, S // T)
     X = torch.complex(rX[: bcs * T * n], iX[: bcs * T * n]).view(bcs, n, T)
     return X.permute(0, 2, 1).reshape(bcs * T, n).contiguous()
 
 
-def _bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, fp64_value):
+def bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, fp64_value):
     N = bc * T
     dev = rA.device
-    rE = torch.empty(N * n, dtype=torch.float64, device=dev)
-    iE = torch.empty_like(rE)
-    rV = torch.empty(N * n * n, dtype=torch.float64, device=dev)
-    iV = torch.empty_like(rV)
+    f64 = dict(dtype=torch.float64, device=dev)
+    rE, iE = torch.empty(N * n, **f64), torch.empty(N * n, **f64)
+    rV, iV = torch.empty(N * n * n, **f64), torch.empty(N * n * n, **f64)
     info = torch.empty(N, dtype=torch.int32, device=dev)
-    berr = torch.empty(N, dtype=torch.float64, device=dev)
+    berr = torch.empty(N, **f64)
     fb = torch.empty(N, dtype=torch.int32, device=dev)
     ws = veig.mixed_workspace(n, bc, T, dev)
 
@@ -208,74 +241,160 @@ def _bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, fp64_value):
         veig.zgeev_mixed(rA, iA, n, bc, T, out=(rE, iE, rV, iV), info=info, berr=berr,
                          fallback=fb, tol=1e-12, workspace=ws, stream=stream)
 
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(a.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / a.steps
+    ms, _ = timed(step, a.steps, a.warmup, stream)
     v = N / (ms * 1e-3)
+    # the complex64 solve alone (the dominant kernel of the mixed path)
+    ms32, _ = timed(lambda: veig.zgeev(rA, iA, n, bc, T, out=(rE, iE, rV, iV), info=info,
+                                       precision="fp32", stream=stream), a.steps, 1, stream)
+    step()
+    torch.cuda.synchronize()
     return {"value": round(v, 1), "unit": "matrices/s", "ms_per_step": round(ms, 4),
             "ratio_vs_fp64": round(v / fp64_value, 3),
             "max_backward_err": float(berr.max()),
             "fp64_fallback_matrices": int(fb.sum()),
             "unconverged_matrices": int((info != 0).sum()),
-            "dtype": "c64 reduction/QR + f64 refinement"}
+            "fp32_kernel_ms": round(ms32, 4),
+            "dtype": "c64 reduction/QR/eigenvectors + f64 Newton refinement (tol 1e-12)"}
 
 
-def _cpu_baseline(rA, iA, n, T, N, target_s):
-    """The reference's own OpenMP code (oracle/_ref/libref.so, four stages with
-    the reflector entries cleared before QR) on a bounded sample of the same
-    inputs; falls back to the oracle restatement when the reference build is
-    absent."""
-    import ctypes
+def _cpu_lib():
+    import helpers
+    lib = helpers.load_ref(exact=False)
+    if lib is not None:
+        return lib, "reference"
+    return helpers.load_oracle(), "port"
+
+
+def _run_cpu(lib, kind, n, tc, rA_np, iA_np):
     import numpy as np
     import helpers
+    S = rA_np.size // (n * n)
+    rE = np.zeros(S * n); iE = np.zeros(S * n)
+    rV = np.zeros(S * n * n); iV = np.zeros(S * n * n)
+    P = helpers.dptr
+    t = time.perf_counter()
+    if kind == "reference":
+        lib.ref_eigensolver_clean(n, S // tc, tc, P(rA_np), P(iA_np), P(rE), P(iE), P(rV), P(iV))
+    else:
+        lib.oracle_eigensolver(n, S // tc, tc, P(rA_np), P(iA_np), P(rE), P(iE), P(rV), P(iV), 0, None)
+    return time.perf_counter() - t
 
-    lib = helpers.load_ref(exact=False)
-    kind = "reference"
-    if lib is None:
-        lib = helpers.load_oracle()
-        kind = "port"
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def cpu_baseline_uniform(rA, iA, n, T, N, target_s):
+    """The reference's own OpenMP code (oracle/_ref/libref.so: its four stages
+    with the reflector entries cleared before QR) on a bounded sample of the
+    same inputs; the oracle restatement where the reference build is absent."""
+    lib, kind = _cpu_lib()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
     def run(S):
         bcs = S // T
         a = rA[: bcs * T * n * n].cpu().numpy().copy()
         b = iA[: bcs * T * n * n].cpu().numpy().copy()
-        rE = np.zeros(S * n); iE = np.zeros(S * n)
-        rV = np.zeros(S * n * n); iV = np.zeros(S * n * n)
-        P = helpers.dptr
-        t = time.perf_counter()
-        if kind == "reference":
-            lib.ref_eigensolver_clean(n, bcs, T, P(a), P(b), P(rE), P(iE), P(rV), P(iV))
-        else:
-            lib.oracle_eigensolver(n, bcs, T, P(a), P(b), P(rE), P(iE), P(rV), P(iV), 0, None)
-        return time.perf_counter() - t
+        return _run_cpu(lib, kind, n, T, a, b)
 
-    S0 = max(T, (4096 // T) * T)
+    S0 = max(T, (min(N, 4096) // T) * T)
     run(S0)  # warm the thread pool
     t0 = run(S0)
     S = int(min(N, max(S0, target_s / max(t0, 1e-6) * S0)))
     S = max(T, (S // T) * T)
     dt = run(S)
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return {"value": round(S / dt, 1), "unit": "matrices/s", "cores": cores, "kind": kind,
             "sample": f"first {S} matrices of the same batch (tile {T}), four stages timed once "
-                      f"after a warm-up call, {dt:.1f} s",
-            "cpu": cpu_model}
+                      f"after a warm-up call, {dt:.1f} s", "cpu": _cpu_model()}
+
+
+def bench_varsize(a, cfg, veig, W, torch, dev, stream):
+    N = cfg["batch"]
+    progress(f"c3: generating {N} inputs")
+    dims, kind = W.c3_dims(N, cfg["nmin"], cfg["nmax"], cfg["seed"], dev)
+    vb = veig.VarBatch(dims.cpu(), device=dev)
+    W.fill_c3(vb, dims, kind, cfg["seed"], dev)
+    torch.cuda.synchronize()
+    g = vb.graph()
+    progress("timing the captured graph")
+    ms, wall = timed(lambda: g.launch(stream), a.steps, a.warmup, stream)
+    value = N / (ms * 1e-3)
+    u, c = torch.unique(dims.cpu(), return_counts=True)
+    flops = float(sum(nominal_flops(int(n)) * int(k) for n, k in zip(u, c)))
+    nbytes = int(sum(alg_bytes(int(n)) * int(k) for n, k in zip(u, c)))
+    rf = roofline(flops, nbytes, ms, "graph: bin kernels + k_zgeev_var<8|16|32>", "varsize_c3")
+    rf["note"] = "whole graph timed (binning + three persistent bin kernels)"
+    info = vb.info
+    # backward error on a sample of every order
+    chk = varsize_check(vb, dims, torch)
+    if not a.no_cpu:
+        progress("timing the reference CPU path on a sample")
+    cpu = None if a.no_cpu else cpu_baseline_varsize(vb, dims, a.cpu_seconds)
+    workload = (f"{N} complex128 matrices, n uniform in [{cfg['nmin']},{cfg['nmax']}]: 50% Gaussian, "
+                f"25% Hermitian, 25% V diag(lam) V^-1 (clusters 1e-6 apart, cond(V)=1e2), seed "
+                f"{cfg['seed']}, per-matrix column-major with offsets; one captured graph per step")
+    return {
+        "metric": f"eigendecomp matrices/s, c3: {workload}", "value": round(value, 1),
+        "unit": "matrices/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload, "name": "c3", "batch": N},
+        "roofline": rf, "cpu_baseline": cpu, "mixed": None,
+        "fp64": {"value": round(value, 1), "ms_per_step": round(ms, 4),
+                 "unconverged_matrices": int((info > 0).sum()), **chk},
+        "wall_s_timed": round(wall, 3),
+    }
+
+
+def varsize_check(vb, dims, torch):
+    worst = 0.0
+    a_off, e_off = vb.a_off, vb.e_off
+    for n in torch.unique(dims).tolist():
+        sel = torch.nonzero(dims == n).flatten()[:256]
+        idx = a_off[sel].unsqueeze(1) + torch.arange(n * n, device=dims.device).unsqueeze(0)
+        A = torch.complex(vb.rA[idx], vb.iA[idx]).view(-1, n, n).transpose(-2, -1)
+        V = torch.complex(vb.rV[idx], vb.iV[idx]).view(-1, n, n).transpose(-2, -1)
+        eidx = e_off[sel].unsqueeze(1) + torch.arange(n, device=dims.device).unsqueeze(0)
+        E = torch.complex(vb.rE[eidx], vb.iE[eidx])
+        R = A @ V - V * E[:, None, :]
+        be = torch.linalg.norm(R, dim=(1, 2)) / (torch.linalg.norm(A, dim=(1, 2)) * torch.linalg.norm(V, dim=(1, 2)))
+        worst = max(worst, float(be.max()))
+    return {"check_sample": "256 per order", "max_backward_err": worst}
+
+
+def cpu_baseline_varsize(vb, dims, target_s):
+    """The reference's code on a sample of the batch: it only takes
+    equal-size batches, so the sample is grouped by order (thread_cnt 1)."""
+    import numpy as np
+    lib, kind = _cpu_lib()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    d = dims.cpu().numpy()
+    a_off = vb.a_off.cpu().numpy()
+    rA = vb.rA.cpu().numpy()
+    iA = vb.iA.cpu().numpy()
+
+    def run(S):
+        tot = 0.0
+        for n in np.unique(d[:S]):
+            sel = np.nonzero(d[:S] == n)[0]
+            idx = (a_off[sel][:, None] + np.arange(n * n)[None, :]).reshape(-1)
+            tot += _run_cpu(lib, kind, int(n), 1, np.ascontiguousarray(rA[idx]), np.ascontiguousarray(iA[idx]))
+        return tot
+
+    S0 = min(len(d), 4096)
+    run(S0)
+    t0 = run(S0)
+    S = int(min(len(d), max(S0, target_s / max(t0, 1e-6) * S0)))
+    dt = run(S)
+    return {"value": round(S / dt, 1), "unit": "matrices/s", "cores": cores, "kind": kind,
+            "sample": f"first {S} matrices of the batch, grouped by order (thread_cnt 1), {dt:.1f} s",
+            "cpu": _cpu_model()}
 
 
 if __name__ == "__main__":

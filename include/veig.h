@@ -35,7 +35,7 @@ extern "C" {
 typedef enum {
   VEIG_OK = 0,
   VEIG_ERR_INVALID = 1,     /* bad argument (negative size, NULL pointer, ...) */
-  VEIG_ERR_UNSUPPORTED = 2, /* order n beyond this build (n > 32 for the lane kernels) */
+  VEIG_ERR_UNSUPPORTED = 2, /* order beyond what the entry point handles */
   VEIG_ERR_HIP = 3,         /* HIP runtime error (launch, copy, allocation) */
   VEIG_ERR_NOMEM = 4        /* device allocation failed */
 } veig_status;
@@ -43,8 +43,10 @@ typedef enum {
 /* flags for veig_zgeev */
 #define VEIG_WRITE_SCHUR 1u /* also store the Schur form T over A (zeros below the diagonal) */
 
-/* Largest order handled by the lane-group kernels. */
-#define VEIG_MAX_DIM 32
+/* Largest order of the fused FP64/complex64 entries (veig_zgeev,
+ * veig_zgeev_fp32, veig_host_eigensolver) and of the other entries. */
+#define VEIG_MAX_DIM 64
+#define VEIG_MAX_DIM_STAGES 32
 
 const char *veig_status_string(int status);
 const char *veig_last_error(void);

@@ -91,7 +91,7 @@ def check_parity(A, E, V, info, Eo, Vo, vec_check=True):
     return be
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 9, 12, 15, 16, 17, 24, 31, 32])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 9, 12, 15, 16, 17, 24, 31, 32, 33, 40, 48, 63, 64])
 def test_gaussian_parity(gpu, n):
     from gpu_util import gpu_eig
     A = gaussian(64, n, 1000 + n)
@@ -109,7 +109,7 @@ def test_tile_layouts(gpu, thread_cnt):
     check_parity(A, E, V, info, Eo, Vo)
 
 
-@pytest.mark.parametrize("n", [4, 8, 16, 24, 32])
+@pytest.mark.parametrize("n", [4, 8, 16, 24, 32, 64])
 def test_hermitian_parity(gpu, n):
     from gpu_util import gpu_eig
     A = hermitian(64, n, 2000 + n)
@@ -122,7 +122,7 @@ def test_hermitian_parity(gpu, n):
     assert np.max(np.abs(E.imag)) <= 1e-12 * np.linalg.norm(A, axis=(1, 2)).max()
 
 
-@pytest.mark.parametrize("n", [8, 16, 24])
+@pytest.mark.parametrize("n", [8, 16, 24, 64])
 def test_constructed_parity(gpu, n):
     from gpu_util import gpu_eig
     A = constructed(64, n, 3000 + n, cond=1e2)
@@ -140,3 +140,13 @@ def test_clustered_backward_error(gpu, n):
     Eo, Vo, _ = oracle_eig(A, 1)
     E, V, info = gpu_eig(A, 1)
     check_parity(A, E, V, info, Eo, Vo, vec_check=False)
+
+
+@pytest.mark.parametrize("thread_cnt", [1, 4])
+def test_order_64_tiles(gpu, thread_cnt):
+    # one matrix per wavefront, eigenvector solves in LDS
+    from gpu_util import gpu_eig
+    A = gaussian(32, 64, 77)
+    Eo, Vo, _ = oracle_eig(A, thread_cnt)
+    E, V, info = gpu_eig(A, thread_cnt)
+    check_parity(A, E, V, info, Eo, Vo)

@@ -240,12 +240,12 @@ const char *veig_status_string(int status) {
 
 const char *veig_last_error(void) { return last_error().c_str(); }
 
-const char *veig_version(void) { return "veig 0.1 gfx950 lane-group kernels (n <= 32)"; }
+const char *veig_version(void) { return "veig 0.2 gfx950 lane-group kernels (fused n <= 64, stages n <= 32)"; }
 
 int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
                double *rE, double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
                uint32_t flags, void *stream) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV); VEIG_REQUIRE(iV);
   FusedArgs a{rA, iA, rE, iE, rV, iV, info, make_geom(dim, block_cnt, thread_cnt),
@@ -256,7 +256,7 @@ int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, d
 int veig_zgeev_fp32(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA,
                     const double *iA, double *rE, double *iE, double *rV, double *iV,
                     int32_t *info, int32_t itmax, void *stream) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV); VEIG_REQUIRE(iV);
   FusedArgs a{const_cast<double *>(rA), const_cast<double *>(iA), rE, iE, rV, iV, info,
@@ -380,7 +380,7 @@ int veig_rep_vector_soa(int64_t dim, int64_t bc, int64_t tc, const double *v, do
 // ---------------------------------------------------------------------------
 int veig_host_eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA,
                           double *rE, double *iE, double *rV, double *iV) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   const size_t sv = (size_t)dim * block_cnt * thread_cnt, sm = sv * dim;
   DevBuf dra, dia, dre, die, drv, div;

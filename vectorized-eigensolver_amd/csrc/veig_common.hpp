@@ -49,17 +49,17 @@ struct Geom {
   __device__ __forceinline__ int64_t ve(int r) const { return (int64_t)r * T; }
 };
 
-inline int check_geom(int64_t dim, int64_t block_cnt, int64_t thread_cnt) {
+inline int check_geom(int64_t dim, int64_t block_cnt, int64_t thread_cnt, int64_t max_dim = VEIG_MAX_DIM_STAGES) {
   if (dim < 0 || block_cnt < 0 || thread_cnt < 1)
     return fail(VEIG_ERR_INVALID, "invalid batch geometry (dim < 0, block_cnt < 0 or thread_cnt < 1)");
-  if (dim > VEIG_MAX_DIM)
-    return fail(VEIG_ERR_UNSUPPORTED, "order " + std::to_string(dim) + " exceeds VEIG_MAX_DIM (" +
-                                          std::to_string(VEIG_MAX_DIM) + ")");
+  if (dim > max_dim)
+    return fail(VEIG_ERR_UNSUPPORTED, "order " + std::to_string(dim) + " exceeds " +
+                                          std::to_string(max_dim) + " for this entry point");
   if (block_cnt > 0 && thread_cnt > INT64_MAX / block_cnt) return fail(VEIG_ERR_INVALID, "batch size overflows");
   return VEIG_OK;
 }
 
-inline int nb_for(int64_t n) { return n <= 8 ? 8 : (n <= 16 ? 16 : 32); }
+inline int nb_for(int64_t n) { return n <= 8 ? 8 : (n <= 16 ? 16 : (n <= 32 ? 32 : 64)); }
 
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -198,6 +198,24 @@ __global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
   gehrd_q<R, NB>(g, z, nmax, true, false, nullptr);
   const int nfail = lahqr<R, NB>(g, z, a.itmax, false);
   if (a.flags & VEIG_WRITE_SCHUR) store_rows<R, NB>(g, a.G, base, a.rA, a.iA);
+  if constexpr (NB == 64) {
+    // one matrix per wavefront: eigenvector solves in LDS, V streamed out
+    static_assert(!WANT_W, "no complex64 inverse at n > 32");
+    const cx<R> lam = trevc_solve_lds<R, NB>(g, nmax);
+    backtransform_lds<R, NB>(g, z, nmax, [&](int k, cx<R> v) {
+      if (g.r < g.n) {
+        const int64_t idx = base + a.G.el(g.r, k);
+        a.rV[idx] = (double)v.re;
+        a.iV[idx] = (double)v.im;
+      }
+    });
+    if (g.r < g.n) {
+      a.rE[vbase + a.G.ve(g.r)] = (double)lam.re;
+      a.iE[vbase + a.G.ve(g.r)] = (double)lam.im;
+    }
+    if (a.info && valid && g.r == 0) a.info[m] = nfail;
+    return;
+  }
   const cx<R> lam = trevc_solve<R, NB>(g, nmax);
   if constexpr (WANT_W) {
     cx<R> w[NB];
@@ -230,7 +248,14 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
   switch (nb) {
     case 8: hipLaunchKernelGGL((k_zgeev<R, 8, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
     case 16: hipLaunchKernelGGL((k_zgeev<R, 16, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_zgeev<R, 32, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_zgeev<R, 32, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
+    default:
+      if constexpr (!WANT_W) {
+        hipLaunchKernelGGL((k_zgeev<R, 64, false>), dim3((unsigned)grid), dim3(64), 0, s, a);
+      } else {
+        return fail(VEIG_ERR_UNSUPPORTED, "mixed precision handles orders up to 32");
+      }
+      break;
   }
   VEIG_HIP(hipGetLastError());
   return VEIG_OK;

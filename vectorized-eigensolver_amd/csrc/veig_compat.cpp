@@ -1,0 +1,191 @@
+// C++ drop-in shim: the reference's host API names (utils.h:37-65,
+// Toolbox-def.hpp:5-22) over the C ABI of libveig.so.  See
+// include/veig_compat.hpp for the semantics.
+#include "veig_compat.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <vector>
+
+namespace {
+
+void check(int st, const char *what) {
+  if (st != VEIG_OK) throw veig::Error(st, std::string(what) + ": " + veig_last_error());
+}
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw veig::Error(VEIG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// device copy of a host plane; copied back on request
+struct Dev {
+  double *d = nullptr;
+  size_t n = 0;
+  Dev(const double *h, size_t count) : n(count) {
+    if (!n) return;
+    hip_check(hipMalloc(&d, n * sizeof(double)), "hipMalloc");
+    if (h) hip_check(hipMemcpy(d, h, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  }
+  void back(double *h) const {
+    if (n) hip_check(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  }
+  ~Dev() {
+    if (d) (void)hipFree(d);
+  }
+  Dev(const Dev &) = delete;
+  Dev &operator=(const Dev &) = delete;
+};
+
+size_t nmat(int bc, int tc) { return (size_t)(bc > 0 ? bc : 0) * (size_t)(tc > 0 ? tc : 0); }
+size_t msize(int dim, int bc, int tc) { return nmat(bc, tc) * (size_t)(dim > 0 ? dim : 0) * (dim > 0 ? dim : 0); }
+size_t vsize(int dim, int bc, int tc) { return nmat(bc, tc) * (size_t)(dim > 0 ? dim : 0); }
+
+void sync() { hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// layout helpers (host loops, utils.cpp:85-157)
+// ---------------------------------------------------------------------------
+void rearrangeMatrices(int dim, int block_cnt, int thread_cnt, double *in, double *out) {
+  const size_t n2 = (size_t)dim * dim;
+  for (size_t b = 0; b < (size_t)block_cnt; ++b)
+    for (size_t t = 0; t < (size_t)thread_cnt; ++t)
+      for (size_t e = 0; e < n2; ++e)
+        out[b * thread_cnt * n2 + e * thread_cnt + t] = in[(b * thread_cnt + t) * n2 + e];
+}
+
+void arrangeMatricesAoS(int dim, int block_cnt, int thread_cnt, double *in, double *out) {
+  const size_t n2 = (size_t)dim * dim;
+  for (size_t b = 0; b < (size_t)block_cnt; ++b)
+    for (size_t t = 0; t < (size_t)thread_cnt; ++t)
+      for (size_t e = 0; e < n2; ++e)
+        out[(b * thread_cnt + t) * n2 + e] = in[b * thread_cnt * n2 + e * thread_cnt + t];
+}
+
+void repMatrixSoA(int dim, int block_cnt, int thread_cnt, double *matrix, double *out) {
+  const size_t n2 = (size_t)dim * dim;
+  for (size_t e = 0; e < n2; ++e)
+    for (size_t b = 0; b < (size_t)block_cnt; ++b)
+      for (size_t t = 0; t < (size_t)thread_cnt; ++t) out[b * thread_cnt * n2 + t + e * thread_cnt] = matrix[e];
+}
+
+void repVectorSoA(int dim, int block_cnt, int thread_cnt, double *vec, double *out) {
+  for (size_t e = 0; e < (size_t)dim; ++e)
+    for (size_t b = 0; b < (size_t)block_cnt; ++b)
+      for (size_t t = 0; t < (size_t)thread_cnt; ++t) out[b * thread_cnt * dim + t + e * thread_cnt] = vec[e];
+}
+
+// ---------------------------------------------------------------------------
+// host API
+// ---------------------------------------------------------------------------
+void call_zlahqr_new(int dim, int block_cnt, int thread_cnt, double *rE, double *iE, double *rM,
+                     double *iM, double *rS, double *iS) {
+  check(veig_host_zlahqr_new(dim, block_cnt, thread_cnt, rE, iE, rM, iM, rS, iS), "call_zlahqr_new");
+}
+
+void call_zgemv(bool hermitian, int dim, int block_cnt, int thread_cnt, double *rM, double *iM,
+                double *rx, double *ix, double *ry, double *iy, double rbeta, double ibeta) {
+  check(veig_host_zgemv(hermitian ? 1 : 0, dim, block_cnt, thread_cnt, rM, iM, rx, ix, ry, iy, rbeta,
+                        ibeta),
+        "call_zgemv");
+}
+
+void call_ztrevc(int dim, int block_cnt, int thread_cnt, double *rT, double *iT, double *rR,
+                 double *iR) {
+  check(veig_host_ztrevc(dim, block_cnt, thread_cnt, rT, iT, rR, iR), "call_ztrevc");
+}
+
+void call_zgehrd(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rtau,
+                 double *itau) {
+  check(veig_host_zgehrd(dim, block_cnt, thread_cnt, rA, iA, rtau, itau), "call_zgehrd");
+}
+
+void call_zunghr(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rtau,
+                 double *itau) {
+  check(veig_host_zunghr(dim, block_cnt, thread_cnt, rA, iA, rtau, itau), "call_zunghr");
+}
+
+void call_backsubstitution(int dim, int block_cnt, int thread_cnt, double *rT, double *iT,
+                           double *rb, double *ib) {
+  check(veig_host_backsubstitution(dim, block_cnt, thread_cnt, rT, iT, rb, ib), "call_backsubstitution");
+}
+
+void hessenberg(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rQ,
+                double *iQ) {
+  check(veig_host_hessenberg(dim, block_cnt, thread_cnt, rA, iA, rQ, iQ), "hessenberg");
+}
+
+void eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rE,
+                 double *iE, double *rV, double *iV) {
+  check(veig_host_eigensolver(dim, block_cnt, thread_cnt, rA, iA, rE, iE, rV, iV), "eigensolver");
+}
+
+// ---------------------------------------------------------------------------
+// batch entry points
+// ---------------------------------------------------------------------------
+void test_zgemv(bool hermitian, double *rA, double *iA, double *rx, double *ix, double rbeta,
+                double ibeta, double *ry, double *iy, int dim, int block_cnt, int thread_cnt) {
+  call_zgemv(hermitian, dim, block_cnt, thread_cnt, rA, iA, rx, ix, ry, iy, rbeta, ibeta);
+}
+
+void test_ztrevc(double *rV, double *iV, double *rZ, double *iZ, double *, double *, int dim, int,
+                 int block_cnt, int thread_cnt) {
+  call_ztrevc(dim, block_cnt, thread_cnt, rV, iV, rZ, iZ);
+}
+
+void zlahqr_new(double *rE, double *iE, double *rV, double *iV, double *rZ, double *iZ, int dim,
+                int, int block_cnt, int thread_cnt) {
+  if (dim <= 0 || block_cnt <= 0 || thread_cnt <= 0) {
+    check(veig_zlahqr(dim, block_cnt, thread_cnt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, 30, nullptr),
+          "zlahqr_new");
+    return;
+  }
+  const size_t sm = msize(dim, block_cnt, thread_cnt), sv = vsize(dim, block_cnt, thread_cnt);
+  Dev h1(rV, sm), h2(iV, sm), z1(rZ, sm), z2(iZ, sm), e1(nullptr, sv), e2(nullptr, sv);
+  check(veig_zlahqr(dim, block_cnt, thread_cnt, e1.d, e2.d, h1.d, h2.d, z1.d, z2.d, nullptr, 30, nullptr),
+        "zlahqr_new");
+  sync();
+  h1.back(rV); h2.back(iV); z1.back(rZ); z2.back(iZ); e1.back(rE); e2.back(iE);
+}
+
+void test_ident(double *rZ, double *iZ, int dim, int block_cnt, int thread_cnt) {
+  const size_t n2 = (size_t)dim * dim;
+  for (size_t b = 0; b < (size_t)block_cnt; ++b)
+    for (size_t t = 0; t < (size_t)thread_cnt; ++t)
+      for (size_t c = 0; c < (size_t)dim; ++c)
+        for (size_t r = 0; r < (size_t)dim; ++r) {
+          const size_t idx = b * thread_cnt * n2 + (r + c * dim) * thread_cnt + t;
+          rZ[idx] = r == c ? 1.0 : 0.0;
+          iZ[idx] = 0.0;
+        }
+}
+
+void test_diag(double *rM, double *iM, double real, double imag, int dim, int block_cnt,
+               int thread_cnt) {
+  const size_t n2 = (size_t)dim * dim;
+  for (size_t b = 0; b < (size_t)block_cnt; ++b)
+    for (size_t t = 0; t < (size_t)thread_cnt; ++t)
+      for (size_t j = 0; j < (size_t)dim; ++j) {
+        const size_t idx = b * thread_cnt * n2 + (j + j * dim) * thread_cnt + t;
+        rM[idx] = real;
+        iM[idx] = imag;
+      }
+}
+
+void test_zgehrd(double *rA, double *iA, double *rtau, double *itau, double *, double *, int dim,
+                 int block_cnt, int thread_cnt) {
+  call_zgehrd(dim, block_cnt, thread_cnt, rA, iA, rtau, itau);
+}
+
+void test_zunghr(double *rA, double *iA, double *rtau, double *itau, double *, double *, int dim,
+                 int block_cnt, int thread_cnt) {
+  call_zunghr(dim, block_cnt, thread_cnt, rA, iA, rtau, itau);
+}
+
+void test_backsubstitution(double *rT, double *iT, double *rx, double *ix, int dim, int block_cnt,
+                           int thread_cnt) {
+  call_backsubstitution(dim, block_cnt, thread_cnt, rT, iT, rx, ix);
+}

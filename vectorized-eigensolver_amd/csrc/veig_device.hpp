@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <float.h>
+#include <type_traits>
 
 namespace veig {
 
@@ -164,6 +165,20 @@ __device__ __forceinline__ R group_sum(R x) {
   return x;
 }
 
+// Call f(std::integral_constant<int, k>) for a wavefront-uniform runtime k in
+// [LO, HI): a binary tree of scalar branches, so that f can index register
+// arrays with a compile-time constant.
+template <int LO, int HI, typename F>
+__device__ __forceinline__ void uniform_dispatch(int k, F&& f) {
+  if constexpr (HI - LO <= 1) {
+    f(std::integral_constant<int, LO>{});
+  } else {
+    constexpr int MID = (LO + HI) / 2;
+    if (k < MID) uniform_dispatch<LO, MID>(k, f);
+    else uniform_dispatch<MID, HI>(k, f);
+  }
+}
+
 // --------------------------------------------------------------------------
 // per-group context
 // --------------------------------------------------------------------------
@@ -233,15 +248,12 @@ __device__ __forceinline__ void gehrd_q(const Group<R, NB>& g, cx<R> (&z)[NB], i
     }
     // ---- Q <- Q * (I - tau v v^H) (row r of Q in registers) ----
     if (accumulate_q) {
-      cx<R> v[NB];
-#pragma unroll
-      for (int c = 0; c < NB; ++c) v[c] = ld<R>(g.vsh + c);
       cx<R> w = mk<R>(0, 0);
 #pragma unroll
-      for (int c = 0; c < NB; ++c) w = add(w, mul(z[c], v[c]));
+      for (int c = 0; c < NB; ++c) w = add(w, mul(z[c], ld<R>(g.vsh + c)));
       const cx<R> tw = mul(tau, w);
 #pragma unroll
-      for (int c = 0; c < NB; ++c) z[c] = sub(z[c], mulc(tw, v[c]));
+      for (int c = 0; c < NB; ++c) z[c] = sub(z[c], mulc(tw, ld<R>(g.vsh + c)));
     }
     wave_sync();
     // ---- H <- (I - conj(tau) v v^H) H on rows i+1..n-1, columns i+1..n-1 ----
@@ -363,8 +375,9 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
           its = 0;
           if (I < 0) done = true;
         } else {
-          const uint32_t bits = (uint32_t)((bal >> base) & ((NB == 64) ? ~0ull : ((1ull << NB) - 1)));
-          if (bits) L = 31 - __clz(bits);
+          // this group's NB ballot bits (NB = 64 uses all 64)
+          const uint64_t bits = (bal >> base) & ((NB == 64) ? ~0ull : ((1ull << NB) - 1));
+          if (bits) L = 63 - __clzll(bits);
           if (L > 0 && r == L) st<R>(g.at(L, L - 1), mk<R>(0, 0));
           if (L >= I) {
             --I;
@@ -431,6 +444,57 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
     }
 
     // ---- the sweep (Toolbox.hpp:138-335), K = bulge row ----
+    if constexpr (Group<R, NB>::MPW == 1) {
+      for (int K = kmin; K < kmax; ++K) {
+        const bool act = ready && K >= L && K < I;
+        cx<R> b0 = a0, b1 = a1;
+        if (act && K > L) {
+          b0 = ld<R>(g.at(K, K - 1));
+          b1 = ld<R>(g.at(K + 1, K - 1));
+        }
+        cx<R> t1 = mk<R>(0, 0), v2 = mk<R>(0, 0), nb0 = b0;
+        if (act) larfg2(b0, b1, t1, v2, nb0);
+        const R t2 = t1.re * v2.re - t1.im * v2.im;  // Re(t1 * v2)
+        if (act && K > L) {
+          if (r == K) st<R>(g.at(K, K - 1), nb0);
+          if (r == K + 1) st<R>(g.at(K + 1, K - 1), mk<R>(0, 0));
+        }
+        // rows K, K+1 over columns K..n-1 (lane = column)
+        if (act && r >= K && r < n) {
+          L2* p1 = g.at(K, r);
+          L2* p2 = g.at(K + 1, r);
+          cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+          const cx<R> sum = add(cmul(t1, h1), scale(h2, t2));
+          h1 = sub(h1, sum);
+          h2 = sub(h2, mul(sum, v2));
+          st<R>(p1, h1);
+          st<R>(p2, h2);
+        }
+        wave_sync();
+        // columns K, K+1 over rows 0..min(K+2, I) (lane = row)
+        const int i3 = (K + 2) < I ? (K + 2) : I;
+        if (act && r <= i3) {
+          L2* p1 = g.at(r, K);
+          L2* p2 = g.at(r, K + 1);
+          cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+          const cx<R> sum = add(mul(t1, h1), scale(h2, t2));
+          h1 = sub(h1, sum);
+          h2 = sub(h2, mulc(sum, v2));
+          st<R>(p1, h1);
+          st<R>(p2, h2);
+        }
+        // Schur vectors: columns K, K+1 of row r (K is wavefront-uniform)
+        if (act) {
+          uniform_dispatch<0, NB - 1>(K, [&](auto kc) {
+            constexpr int KK = decltype(kc)::value;
+            const cx<R> sum = add(mul(t1, z[KK]), scale(z[KK + 1], t2));
+            z[KK] = sub(z[KK], sum);
+            z[KK + 1] = sub(z[KK + 1], mulc(sum, v2));
+          });
+        }
+        wave_sync();
+      }
+    } else {
 #pragma unroll
     for (int K = 0; K < NB - 1; ++K) {
       if (K >= kmin && K < kmax) {
@@ -480,20 +544,28 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
         wave_sync();
       }
     }
+    }
 
     // ---- make H(I, I-1) real (Toolbox.hpp:337-367) ----
     if (ready) {
       const cx<R> h = ld<R>(g.at(I, I - 1));
-      if (h.im != R(0)) {
+      if (h.im != R(0) && abs2(h) > R(0)) {  // |h|^2 may underflow: leave such h alone
         const R id = frsq(abs2(h));
         const R d = abs2(h) * id;
         const cx<R> zz = mk(h.re * id, -h.im * id);  // conj(h/|h|)
         if (r == I) st<R>(g.at(I, I - 1), mk<R>(d, 0));
         if (r > I && r < n) st<R>(g.at(I, r), mul(ld<R>(g.at(I, r)), zz));
         if (r < I) st<R>(g.at(r, I), mulc(ld<R>(g.at(r, I)), zz));
+        if constexpr (Group<R, NB>::MPW == 1) {
+          uniform_dispatch<0, NB>(I, [&](auto kc) {
+            constexpr int KK = decltype(kc)::value;
+            z[KK] = mulc(z[KK], zz);
+          });
+        } else {
 #pragma unroll
-        for (int c = 1; c < NB; ++c)
-          if (c == I) z[c] = mulc(z[c], zz);
+          for (int c = 1; c < NB; ++c)
+            if (c == I) z[c] = mulc(z[c], zz);
+        }
       }
       ++its;
     }
@@ -605,6 +677,60 @@ __device__ __forceinline__ void backtransform(const Group<R, NB>& g, cx<R> (&z)[
       for (int j = 0; j < k; ++j) acc = add(acc, mul(z[j], ld<R>(g.at(j, k))));
       z[k] = acc;
     }
+  }
+}
+
+// --------------------------------------------------------------------------
+// The same eigenvector solve for one matrix per wavefront (n up to 64), with
+// the right-hand sides kept in LDS instead of registers: lane k keeps
+// y = -x of its eigenvector in column k of the tile (rows < k).  At step j
+// lanes k > j read column j of T, which lane j only starts to overwrite at
+// step j-1, so T and the solutions share the tile.
+// --------------------------------------------------------------------------
+template <typename R, int NB>
+__device__ __forceinline__ cx<R> trevc_solve_lds(const Group<R, NB>& g, int nmax) {
+  const int r = g.r, n = g.n;
+  const R ulp = limits<R>::eps;
+  const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
+  const bool in = r < n;
+  const cx<R> lam = in ? ld<R>(g.at(r, r)) : mk<R>(0, 0);
+  const R smin = fmax(ulp * cabs1(lam), smlnum);
+  for (int j = nmax - 2; j >= 0; --j) {
+    const bool act = in && j < r;
+    cx<R> d = sub(ld<R>(g.at(j, j)), lam);
+    if (cabs1(d) < smin) d = mk<R>(smin, 0);
+    cx<R> yj = mk<R>(0, 0);
+    if (act) {
+      yj = mul(ld<R>(g.at(j, r)), inv(d));
+      st<R>(g.at(j, r), yj);
+    }
+    for (int i = 0; i < j; ++i) {
+      const cx<R> tij = ld<R>(g.at(i, j));
+      if (act) st<R>(g.at(i, r), sub(ld<R>(g.at(i, r)), mul(tij, yj)));
+    }
+    wave_sync();
+  }
+  return lam;
+}
+
+// Row r of V = Z X from the tile left by trevc_solve_lds, one column at a
+// time, stored straight to the output (column k of V = Z [x; 1; 0]).
+template <typename R, int NB, typename Store>
+__device__ __forceinline__ void backtransform_lds(const Group<R, NB>& g, const cx<R> (&z)[NB],
+                                                  int nmax, Store&& store) {
+  const int r = g.r;
+  // tile column r becomes -X(:, r): y above the diagonal, -1 on it, 0 below
+  if (r < g.n) {
+    st<R>(g.at(r, r), mk<R>(-1, 0));
+    for (int i = r + 1; i < g.n; ++i) st<R>(g.at(i, r), mk<R>(0, 0));
+  }
+  wave_sync();
+  for (int k = 0; k < nmax; ++k) {
+    cx<R> acc = mk<R>(0, 0);
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (j < nmax) acc = add(acc, mul(z[j], ld<R>(g.at(j, k))));
+    store(k, mk(-acc.re, -acc.im));
   }
 }
 

@@ -8,11 +8,12 @@ with torch (fixed seeds), in the layouts the entry points take.
       cond(V) = 1e2                                          per-matrix offsets
   c4  524,288 complex128 24x24, A = V diag(lam) V^-1, lam iid CN(0,1),
       cond(V) log-uniform in [1e1, 1e6]                      tiled SoA
-  c5  65,536 complex128 64x64, re/im iid N(0,1)              tiled SoA
+  c5  65,536 complex128 64x64, re/im iid N(0,1)              tile 1 (one matrix
+      after another: a wavefront per matrix reads and writes whole columns)
 
-V is built as U diag(sigma) W^H with Haar-unitary U, W (QR of complex
-Gaussians with the phases of R's diagonal folded in) and sigma log-spaced
-from 1 down to 1/cond.
+V is built as U diag(sigma) W^H with random unitary U, W (products of random
+Householder reflections and a random diagonal phase) and sigma log-spaced
+from 1 down to 1/cond; V^{-1} = W diag(1/sigma) U^H.
 """
 from __future__ import annotations
 
@@ -25,7 +26,7 @@ CONFIGS = {
     "c2": dict(n=16, batch=1 << 20, seed=1002),
     "c3": dict(nmin=4, nmax=32, batch=262144, seed=1003),
     "c4": dict(n=24, batch=524288, seed=1004),
-    "c5": dict(n=64, batch=65536, seed=1005),
+    "c5": dict(n=64, batch=65536, seed=1005, tile=1),
 }
 
 
@@ -45,25 +46,36 @@ def gaussian_soa(N: int, n: int, tile: int, seed: int, device="cuda"):
     return rA, iA
 
 
-def haar(N: int, n: int, g, device):
-    Z = torch.complex(torch.randn(N, n, n, dtype=torch.float64, device=device, generator=g),
-                      torch.randn(N, n, n, dtype=torch.float64, device=device, generator=g))
-    Q, R = torch.linalg.qr(Z)
-    d = torch.diagonal(R, dim1=-2, dim2=-1)
-    return Q * (d / d.abs()).unsqueeze(-2)
+def random_unitary(N: int, n: int, g, device, reflectors: int = 4):
+    """Random unitary matrices: a product of `reflectors` Householder
+    reflections with random directions times a random diagonal phase
+    (batched rank-1 updates, no factorisation)."""
+    ph = torch.rand(N, n, dtype=torch.float64, device=device, generator=g) * (2 * math.pi)
+    Q = torch.diag_embed(torch.polar(torch.ones_like(ph), ph))
+    for _ in range(reflectors):
+        u = torch.complex(torch.randn(N, n, dtype=torch.float64, device=device, generator=g),
+                          torch.randn(N, n, dtype=torch.float64, device=device, generator=g))
+        u = u / torch.linalg.vector_norm(u, dim=1, keepdim=True)
+        Q = Q - 2 * u.unsqueeze(2) * (u.conj().unsqueeze(1) @ Q)
+    return Q
 
 
 def conditioned(N: int, n: int, cond, g, device, lam=None):
-    """A = V diag(lam) V^{-1} with cond(V) = cond (scalar or per-matrix tensor)."""
-    U, W = haar(N, n, g, device), haar(N, n, g, device)
+    """A = V diag(lam) V^{-1} with V = U diag(sigma) W^H, cond(V) = cond
+    (scalar or per-matrix tensor); V^{-1} = W diag(1/sigma) U^H is formed
+    directly, so no solve is needed."""
+    U, W = random_unitary(N, n, g, device), random_unitary(N, n, g, device)
     c = torch.as_tensor(cond, dtype=torch.float64, device=device).reshape(-1, 1)
     t = torch.linspace(0, 1, n, dtype=torch.float64, device=device).reshape(1, -1)
-    sig = torch.exp(-t * torch.log(c))                         # 1 .. 1/cond
-    V = (U * sig.unsqueeze(1).to(U.dtype)) @ W.conj().transpose(-2, -1)
+    sig = torch.exp(-t * torch.log(c)).expand(N, n)             # 1 .. 1/cond
     if lam is None:
         lam = torch.complex(torch.randn(N, n, dtype=torch.float64, device=device, generator=g),
                             torch.randn(N, n, dtype=torch.float64, device=device, generator=g)) / math.sqrt(2)
-    return torch.linalg.solve(V.transpose(-2, -1), (V * lam.unsqueeze(-2)).transpose(-2, -1)).transpose(-2, -1)
+    WH = W.conj().transpose(-2, -1).contiguous()
+    M = (WH * lam.unsqueeze(-2)) @ W                            # W^H diag(lam) W
+    left = (U * sig.unsqueeze(1).to(U.dtype)).contiguous()      # U diag(sigma)
+    right = (U.conj().transpose(-2, -1) / sig.unsqueeze(2).to(U.dtype)).contiguous()  # diag(1/sigma) U^H
+    return left @ M @ right
 
 
 def clustered_lam(N: int, n: int, g, device, spacing=1e-6):
