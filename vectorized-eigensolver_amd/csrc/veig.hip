@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -14,6 +15,18 @@ namespace veig {
 std::string &last_error() {
   static thread_local std::string s;
   return s;
+}
+
+int slots_for(bool fp32, int nb) {
+  // more matrices per wavefront pay while the slot registers fit
+  static const int env = [] {
+    const char *e = getenv("VEIG_SLOTS");
+    return e ? atoi(e) : 0;
+  }();
+  if (env == 1 || env == 2 || env == 4) return env;
+  (void)fp32;
+  (void)nb;
+  return 1;
 }
 }  // namespace veig
 
@@ -49,7 +62,7 @@ __global__ __launch_bounds__(64) void k_stage(StageArgs a) {
   const int64_t vbase = valid ? a.G.vec_base(m) : 0;
   const int nmax = a.G.n;
   wave_sync();
-  cx<R> z[NB];
+  cx<R> z[1][NB];
   if (a.mode == ST_GEHRD || a.mode == ST_HESS) {
     load_rows<R, NB>(g, a.G, base, a.rA, a.iA, NB);
     identity_rows<R, NB>(g, z);
@@ -82,10 +95,10 @@ __global__ __launch_bounds__(64) void k_stage(StageArgs a) {
       wave_sync();
       cx<R> w = mk<R>(0, 0);
 #pragma unroll
-      for (int c = 0; c < NB; ++c) w = add(w, mul(z[c], ld<R>(g.vsh + c)));
+      for (int c = 0; c < NB; ++c) w = add(w, mul(z[0][c], ld<R>(g.vsh + c)));
       const cx<R> tw = mul(tau, w);
 #pragma unroll
-      for (int c = 0; c < NB; ++c) z[c] = sub(z[c], mulc(tw, ld<R>(g.vsh + c)));
+      for (int c = 0; c < NB; ++c) z[0][c] = sub(z[0][c], mulc(tw, ld<R>(g.vsh + c)));
     }
     store_row_regs<R, NB>(g, a.G, base, a.rA, a.iA, z);
   } else if (a.mode == ST_LAHQR) {
@@ -105,7 +118,7 @@ __global__ __launch_bounds__(64) void k_stage(StageArgs a) {
     load_rows<R, NB>(g, a.G, base, a.rA, a.iA, 0);  // upper triangle only
     load_row_regs<R, NB>(g, a.G, base, a.rB, a.iB, z);
     wave_sync();
-    cx<R> lam;
+    cx<R> lam[1];
     trevc_backtransform<R, NB>(g, z, lam, nmax);
     store_row_regs<R, NB>(g, a.G, base, a.rB, a.iB, z);
   }

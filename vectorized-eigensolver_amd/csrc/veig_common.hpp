@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "veig_device.hpp"
 #include "../../include/veig.h"
@@ -76,91 +77,128 @@ __device__ __forceinline__ int64_t xcd_block() {
 // --------------------------------------------------------------------------
 // common kernel prologue: lane group setup and LDS tile
 // --------------------------------------------------------------------------
-template <typename R, int NB>
+template <typename R, int NB, int P = 1>
 struct Tiles {
   using L2 = typename lds2<R>::type;
-  static constexpr int MPW = 64 / NB;
+  static constexpr int MPW = Group<R, NB, P>::MPW;
   static constexpr int LD = NB + 1;
   L2 H[MPW * NB * LD];
   L2 v[MPW * NB];
 };
 
-template <typename R, int NB>
-__device__ __forceinline__ Group<R, NB> make_group(Tiles<R, NB> &sm, int n_of_matrix) {
-  Group<R, NB> g;
+template <typename R, int NB, int P = 1>
+__device__ __forceinline__ Group<R, NB, P> make_group(Tiles<R, NB, P> &sm, int n_of_matrix) {
+  using G = Group<R, NB, P>;
+  G g;
   const int lane = threadIdx.x;
-  g.mi = lane / NB;
-  g.r = lane % NB;
+  g.mi = lane / G::LG;
+  g.r = lane % G::LG;
   g.n = n_of_matrix;
-  g.H = sm.H + g.mi * NB * Group<R, NB>::LD;
+  g.H = sm.H + g.mi * NB * G::LD;
   g.vsh = sm.v + g.mi * NB;
   // zero the whole tile: entries outside the matrix (and below the
   // subdiagonal when only a Hessenberg part is loaded) read as exact zeros
 #pragma unroll
-  for (int c = 0; c < NB; ++c) st<R>(g.at(g.r, c), mk<R>(0, 0));
-  st<R>(g.vsh + g.r, mk<R>(0, 0));
+  for (int p = 0; p < P; ++p) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) st<R>(g.at(g.idx(p), c), mk<R>(0, 0));
+    st<R>(g.vsh + g.idx(p), mk<R>(0, 0));
+  }
   return g;
 }
 
-// lane r loads row r of a full matrix into the tile
-template <typename R, int NB>
-__device__ __forceinline__ void load_rows(const Group<R, NB> &g, const Geom &G, int64_t base,
+// each lane loads its rows of a full matrix into the tile (entries with
+// row <= col + lower_band)
+template <typename R, int NB, int P>
+__device__ __forceinline__ void load_rows(const Group<R, NB, P> &g, const Geom &G, int64_t base,
                                           const double *re, const double *im, int lower_band) {
-  if (g.r < g.n) {
-    for (int c = 0; c < g.n; ++c) {
-      if (g.r <= c + lower_band) {
-        const int64_t idx = base + G.el(g.r, c);
-        st<R>(g.at(g.r, c), mk<R>((R)re[idx], (R)im[idx]));
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int row = g.idx(p);
+    if (row < g.n) {
+      for (int c = 0; c < g.n; ++c) {
+        if (row <= c + lower_band) {
+          const int64_t idx = base + G.el(row, c);
+          st<R>(g.at(row, c), mk<R>((R)re[idx], (R)im[idx]));
+        }
       }
     }
   }
 }
 
-// lane r loads row r of a matrix into registers
-template <typename R, int NB>
-__device__ __forceinline__ void load_row_regs(const Group<R, NB> &g, const Geom &G, int64_t base,
-                                              const double *re, const double *im, cx<R> (&z)[NB]) {
+// each lane loads its rows of a matrix into registers
+template <typename R, int NB, int P>
+__device__ __forceinline__ void load_row_regs(const Group<R, NB, P> &g, const Geom &G, int64_t base,
+                                              const double *re, const double *im, cx<R> (&z)[P][NB]) {
 #pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    z[c] = mk<R>(0, 0);
-    if (g.r < g.n && c < g.n) {
-      const int64_t idx = base + G.el(g.r, c);
-      z[c] = mk<R>((R)re[idx], (R)im[idx]);
+  for (int p = 0; p < P; ++p) {
+    const int row = g.idx(p);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      z[p][c] = mk<R>(0, 0);
+      if (row < g.n && c < g.n) {
+        const int64_t idx = base + G.el(row, c);
+        z[p][c] = mk<R>((R)re[idx], (R)im[idx]);
+      }
     }
   }
 }
 
-template <typename R, int NB>
-__device__ __forceinline__ void store_row_regs(const Group<R, NB> &g, const Geom &G, int64_t base,
-                                               double *re, double *im, const cx<R> (&z)[NB]) {
+template <typename R, int NB, int P>
+__device__ __forceinline__ void store_row_regs(const Group<R, NB, P> &g, const Geom &G, int64_t base,
+                                               double *re, double *im, const cx<R> (&z)[P][NB]) {
 #pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    if (g.r < g.n && c < g.n) {
-      const int64_t idx = base + G.el(g.r, c);
-      re[idx] = (double)z[c].re;
-      im[idx] = (double)z[c].im;
+  for (int p = 0; p < P; ++p) {
+    const int row = g.idx(p);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      if (row < g.n && c < g.n) {
+        const int64_t idx = base + G.el(row, c);
+        re[idx] = (double)z[p][c].re;
+        im[idx] = (double)z[p][c].im;
+      }
     }
   }
 }
 
-// lane r stores row r of the tile (full matrix)
-template <typename R, int NB>
-__device__ __forceinline__ void store_rows(const Group<R, NB> &g, const Geom &G, int64_t base,
+// each lane stores its rows of the tile (full matrix)
+template <typename R, int NB, int P>
+__device__ __forceinline__ void store_rows(const Group<R, NB, P> &g, const Geom &G, int64_t base,
                                            double *re, double *im) {
-  if (g.r < g.n) {
-    for (int c = 0; c < g.n; ++c) {
-      const int64_t idx = base + G.el(g.r, c);
-      const cx<R> v = ld<R>(g.at(g.r, c));
-      re[idx] = (double)v.re;
-      im[idx] = (double)v.im;
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int row = g.idx(p);
+    if (row < g.n) {
+      for (int c = 0; c < g.n; ++c) {
+        const int64_t idx = base + G.el(row, c);
+        const cx<R> v = ld<R>(g.at(row, c));
+        re[idx] = (double)v.re;
+        im[idx] = (double)v.im;
+      }
     }
   }
 }
 
-template <typename R, int NB>
-__device__ __forceinline__ void identity_rows(const Group<R, NB> &g, cx<R> (&z)[NB]) {
+// each lane's eigenvalues lam[p] = T(idx(p), idx(p)) to the E planes
+template <typename R, int NB, int P>
+__device__ __forceinline__ void store_eigenvalues(const Group<R, NB, P> &g, const Geom &G, int64_t vbase,
+                                                  double *re, double *im, const cx<R> (&lam)[P]) {
 #pragma unroll
-  for (int c = 0; c < NB; ++c) z[c] = mk<R>(c == g.r ? R(1) : R(0), 0);
+  for (int p = 0; p < P; ++p) {
+    const int k = g.idx(p);
+    if (k < g.n) {
+      re[vbase + G.ve(k)] = (double)lam[p].re;
+      im[vbase + G.ve(k)] = (double)lam[p].im;
+    }
+  }
+}
+
+template <typename R, int NB, int P>
+__device__ __forceinline__ void identity_rows(const Group<R, NB, P> &g, cx<R> (&z)[P][NB]) {
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int c = 0; c < NB; ++c) z[p][c] = mk<R>(c == g.idx(p) ? R(1) : R(0), 0);
 }
 
 // --------------------------------------------------------------------------
@@ -177,30 +215,31 @@ struct FusedArgs {
   float *rW, *iW;       // if set (WANT_W), W = V^{-1} in complex64 planes
 };
 
-template <typename R, int NB, bool WANT_W>
+template <typename R, int NB, int P, bool WANT_W>
 __global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
-  __shared__ Tiles<R, NB> sm;
-  constexpr int MPW = Tiles<R, NB>::MPW;
-  const int64_t m = xcd_block() * MPW + threadIdx.x / NB;
+  __shared__ Tiles<R, NB, P> sm;
+  using G = Group<R, NB, P>;
+  constexpr int MPW = G::MPW;
+  const int64_t m = xcd_block() * MPW + threadIdx.x / G::LG;
   bool valid = m < a.G.nmat;
   if (valid && a.mask) valid = a.mask[m] != 0;
   if (!__any(valid)) return;  // wave-uniform early exit (masked launches)
   const int n = valid ? a.G.n : 0;
-  Group<R, NB> g = make_group<R, NB>(sm, n);
+  G g = make_group<R, NB, P>(sm, n);
   const int64_t base = valid ? a.G.mat_base(m) : 0;
   const int64_t vbase = valid ? a.G.vec_base(m) : 0;
   wave_sync();
-  load_rows<R, NB>(g, a.G, base, a.rA, a.iA, NB);
-  cx<R> z[NB];
-  identity_rows<R, NB>(g, z);
+  load_rows<R, NB, P>(g, a.G, base, a.rA, a.iA, NB);
+  cx<R> z[P][NB];
+  identity_rows<R, NB, P>(g, z);
   wave_sync();
   const int nmax = a.G.n;  // every matrix of a launch has the same order
-  gehrd_q<R, NB>(g, z, nmax, true, false, nullptr);
-  const int nfail = lahqr<R, NB>(g, z, a.itmax, false);
-  if (a.flags & VEIG_WRITE_SCHUR) store_rows<R, NB>(g, a.G, base, a.rA, a.iA);
+  gehrd_q<R, NB, P>(g, z, nmax, true, false, nullptr);
+  const int nfail = lahqr<R, NB, P>(g, z, a.itmax, false);
+  if (a.flags & VEIG_WRITE_SCHUR) store_rows<R, NB, P>(g, a.G, base, a.rA, a.iA);
   if constexpr (NB == 64) {
     // one matrix per wavefront: eigenvector solves in LDS, V streamed out
-    static_assert(!WANT_W, "no complex64 inverse at n > 32");
+    static_assert(!WANT_W && P == 1, "one matrix per wavefront: FP64, one index per lane");
     const cx<R> lam = trevc_solve_lds<R, NB>(g, nmax);
     backtransform_lds<R, NB>(g, z, nmax, [&](int k, cx<R> v) {
       if (g.r < g.n) {
@@ -216,42 +255,70 @@ __global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
     if (a.info && valid && g.r == 0) a.info[m] = nfail;
     return;
   }
-  const cx<R> lam = trevc_solve<R, NB>(g, nmax);
+  cx<R> lam[P];
+  trevc_solve<R, NB, P>(g, nmax, lam);
   if constexpr (WANT_W) {
-    cx<R> w[NB];
-    inverse_vectors<R, NB>(g, z, w, nmax);
-    // column r of W
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      if (g.r < g.n && k < g.n) {
-        const int64_t idx = base + a.G.el(k, g.r);
-        a.rW[idx] = (float)w[k].re;
-        a.iW[idx] = (float)w[k].im;
+    // column idx(p) of W
+    inverse_vectors<R, NB, P>(g, z, nmax, [&](int p, int k, cx<R> w) {
+      const int col = g.idx(p);
+      if (col < g.n && k < g.n) {
+        const int64_t idx = base + a.G.el(k, col);
+        a.rW[idx] = (float)w.re;
+        a.iW[idx] = (float)w.im;
       }
-    }
+    });
   }
-  backtransform<R, NB>(g, z, nmax);
-  store_row_regs<R, NB>(g, a.G, base, a.rV, a.iV, z);
-  if (g.r < g.n) {
-    a.rE[vbase + a.G.ve(g.r)] = (double)lam.re;
-    a.iE[vbase + a.G.ve(g.r)] = (double)lam.im;
-  }
+  backtransform<R, NB, P>(g, z, nmax);
+  store_row_regs<R, NB, P>(g, a.G, base, a.rV, a.iV, z);
+  store_eigenvalues<R, NB, P>(g, a.G, vbase, a.rE, a.iE, lam);
   if (a.info && valid && g.r == 0) a.info[m] = nfail;
+}
+
+// Lane slots per index of the fused kernel, per precision and padded order
+// (P = 1: one lane per index).  VEIG_SLOTS=<p> overrides the table where the
+// kernel exists with that many slots (tuning).
+int slots_for(bool fp32, int nb);
+
+template <typename R, int NB, int P, bool WANT_W>
+inline void launch_one(const FusedArgs &a, hipStream_t s) {
+  constexpr int mpw = Group<R, NB, P>::MPW;
+  const int64_t grid = (a.G.nmat + mpw - 1) / mpw;
+  hipLaunchKernelGGL((k_zgeev<R, NB, P, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a);
 }
 
 template <typename R, bool WANT_W = false>
 int launch_zgeev(const FusedArgs &a, hipStream_t s) {
+  constexpr bool f32 = std::is_same<R, float>::value;
   const int nb = nb_for(a.G.n);
-  const int mpw = 64 / nb;
-  const int64_t grid = (a.G.nmat + mpw - 1) / mpw;
-  if (grid > INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");
+  const int p = slots_for(f32, nb);
+  if (a.G.nmat > (int64_t)INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");
   switch (nb) {
-    case 8: hipLaunchKernelGGL((k_zgeev<R, 8, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((k_zgeev<R, 16, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((k_zgeev<R, 32, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a); break;
+    case 8:
+      if (p == 2) launch_one<R, 8, 2, WANT_W>(a, s);
+      else launch_one<R, 8, 1, WANT_W>(a, s);
+      break;
+    case 16:
+      if constexpr (f32) {
+        if (p == 4) {
+          launch_one<R, 16, 4, WANT_W>(a, s);
+          break;
+        }
+      }
+      if (p == 2) launch_one<R, 16, 2, WANT_W>(a, s);
+      else launch_one<R, 16, 1, WANT_W>(a, s);
+      break;
+    case 32:
+      if constexpr (f32) {
+        if (p == 2) {
+          launch_one<R, 32, 2, WANT_W>(a, s);
+          break;
+        }
+      }
+      launch_one<R, 32, 1, WANT_W>(a, s);
+      break;
     default:
       if constexpr (!WANT_W) {
-        hipLaunchKernelGGL((k_zgeev<R, 64, false>), dim3((unsigned)grid), dim3(64), 0, s, a);
+        launch_one<R, 64, 1, false>(a, s);
       } else {
         return fail(VEIG_ERR_UNSUPPORTED, "mixed precision handles orders up to 32");
       }

@@ -182,37 +182,51 @@ __device__ __forceinline__ void uniform_dispatch(int k, F&& f) {
 // --------------------------------------------------------------------------
 // per-group context
 // --------------------------------------------------------------------------
-template <typename R, int NB>
+// A group of LG = NB/P lanes holds one matrix of order n <= NB; lane q of the
+// group owns the P indices q, q + LG, ..., q + (P-1) LG ("slots").  P = 1 is
+// one lane per index; P = 2 and 4 put 2 or 4 times as many matrices in a
+// wavefront, which spreads the per-step scalar work (reflector generation,
+// shifts, deflation bookkeeping) over more matrices per instruction.
+template <typename R, int NB, int P = 1>
 struct Group {
+  static_assert(NB % P == 0 && 64 % (NB / P) == 0, "lane group must divide the wavefront");
+  static constexpr int LG = NB / P;    // lanes per matrix
   static constexpr int LD = NB + 1;
-  static constexpr int MPW = 64 / NB;  // matrices per wavefront
+  static constexpr int MPW = 64 / LG;  // matrices per wavefront
   using L2 = typename lds2<R>::type;
   L2* H;     // NB x LD complex tile, H(r,c) at H[c*LD + r]
   L2* vsh;   // NB complex scratch (reflector vector)
-  int r;     // lane index within the group (row / column owned)
+  int r;     // lane index within the group (slot 0 index)
   int mi;    // group index within the wavefront
   int n;     // order of this group's matrix (0 if the slot is empty)
+  __device__ __forceinline__ int idx(int p) const { return r + p * LG; }
   __device__ __forceinline__ L2* at(int row, int col) const { return H + col * LD + row; }
 };
 
 // --------------------------------------------------------------------------
 // Hessenberg reduction with forward accumulation of Q (zgehrd + zunghr,
 // Toolbox.hpp:1315-1659).  On exit H holds the upper Hessenberg matrix with
-// a real subdiagonal and exact zeros below it, and z[] holds row r of Q with
-// A = Q H Q^H.  nmax is the wavefront-uniform loop bound (max n in the wave).
-// If keep_reflectors, v(i+2:n) is left in H(i+2:n, i) and tau(i) is returned
-// in tau_out[i] of lane i (zgehrd's output convention).
+// a real subdiagonal and exact zeros below it, and z[p][] holds row idx(p) of
+// Q with A = Q H Q^H.  nmax is the wavefront-uniform loop bound (max n in the
+// wave).  If keep_reflectors, v(i+2:n) is left in H(i+2:n, i) and tau(i) is
+// returned in tau_out[p] of the lane owning index i (zgehrd's convention).
 // --------------------------------------------------------------------------
-template <typename R, int NB>
-__device__ __forceinline__ void gehrd_q(const Group<R, NB>& g, cx<R> (&z)[NB], int nmax,
+template <typename R, int NB, int P>
+__device__ __forceinline__ void gehrd_q(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], int nmax,
                                         bool accumulate_q, bool keep_reflectors, cx<R>* tau_out) {
-  const int r = g.r, n = g.n;
+  constexpr int LG = Group<R, NB, P>::LG;
+  const int n = g.n;
   for (int i = 0; i + 1 < nmax; ++i) {
     // ---- reflector annihilating H(i+2:n, i) (zlarfg, LAPACK sign) ----
-    const bool in = r < n;
-    const cx<R> hri = in ? ld<R>(g.at(r, i)) : mk<R>(0, 0);
-    const R part = (r >= i + 2 && in) ? abs2(hri) : R(0);
-    const R xnorm2 = group_sum<NB>(part);
+    cx<R> hri[P];
+    R part = R(0);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int row = g.idx(p);
+      hri[p] = row < n ? ld<R>(g.at(row, i)) : mk<R>(0, 0);
+      if (row >= i + 2 && row < n) part += abs2(hri[p]);
+    }
+    const R xnorm2 = group_sum<LG>(part);
     const bool live = i + 1 < n;  // this matrix still has a column to reduce
     const cx<R> alpha = live ? ld<R>(g.at(i + 1, i)) : mk<R>(0, 0);
     cx<R> tau = mk<R>(0, 0), scal = mk<R>(0, 0);
@@ -224,45 +238,87 @@ __device__ __forceinline__ void gehrd_q(const Group<R, NB>& g, cx<R> (&z)[NB], i
       tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
       scal = inv(mk(alpha.re - beta, alpha.im));
     }
-    cx<R> vr = mk<R>(0, 0);
-    if (live) {
-      if (r == i + 1) vr = mk<R>(1, 0);
-      else if (r >= i + 2 && in) vr = mul(hri, scal);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int row = g.idx(p);
+      const bool in = row < n;
+      cx<R> vr = mk<R>(0, 0);
+      if (live) {
+        if (row == i + 1) vr = mk<R>(1, 0);
+        else if (row >= i + 2 && in) vr = mul(hri[p], scal);
+      }
+      st<R>(g.vsh + row, vr);
+      if (live && in && row >= i + 1) {
+        if (row == i + 1) st<R>(g.at(row, i), mk<R>(beta, 0));
+        else st<R>(g.at(row, i), keep_reflectors ? vr : mk<R>(0, 0));
+      }
+      if (keep_reflectors && row == i) tau_out[p] = tau;
     }
-    st<R>(g.vsh + r, vr);
-    if (live && in && r >= i + 1) {
-      if (r == i + 1) st<R>(g.at(r, i), mk<R>(beta, 0));
-      else st<R>(g.at(r, i), keep_reflectors ? vr : mk<R>(0, 0));
-    }
-    if (keep_reflectors && r == i) tau_out[0] = tau;
     wave_sync();
     // ---- H <- H * (I - tau v v^H) on rows 0..n-1, columns i+1..n-1 ----
-    if (live && in) {
-      cx<R> w = mk<R>(0, 0);
-      for (int c = i + 1; c < n; ++c) w = add(w, mul(ld<R>(g.at(r, c)), ld<R>(g.vsh + c)));
-      const cx<R> tw = mul(tau, w);
-      for (int c = i + 1; c < n; ++c) {
+    if (live) {
+      cx<R> w[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) w[p] = mk<R>(0, 0);
+      for (int c = i + 1; c < nmax; ++c) {
+        const cx<R> v = ld<R>(g.vsh + c);  // zero beyond n
+#pragma unroll
+        for (int p = 0; p < P; ++p) w[p] = add(w[p], mul(ld<R>(g.at(g.idx(p), c)), v));
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) w[p] = mul(tau, w[p]);
+      for (int c = i + 1; c < nmax; ++c) {
         const cx<R> v = ld<R>(g.vsh + c);
-        st<R>(g.at(r, c), sub(ld<R>(g.at(r, c)), mulc(tw, v)));
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const int row = g.idx(p);
+          if (row < n && c < n) st<R>(g.at(row, c), sub(ld<R>(g.at(row, c)), mulc(w[p], v)));
+        }
       }
     }
-    // ---- Q <- Q * (I - tau v v^H) (row r of Q in registers) ----
+    // ---- Q <- Q * (I - tau v v^H) (rows idx(p) of Q in registers) ----
     if (accumulate_q) {
-      cx<R> w = mk<R>(0, 0);
+      cx<R> w[P];
 #pragma unroll
-      for (int c = 0; c < NB; ++c) w = add(w, mul(z[c], ld<R>(g.vsh + c)));
-      const cx<R> tw = mul(tau, w);
+      for (int p = 0; p < P; ++p) w[p] = mk<R>(0, 0);
 #pragma unroll
-      for (int c = 0; c < NB; ++c) z[c] = sub(z[c], mulc(tw, ld<R>(g.vsh + c)));
+      for (int c = 0; c < NB; ++c) {
+        const cx<R> v = ld<R>(g.vsh + c);
+#pragma unroll
+        for (int p = 0; p < P; ++p) w[p] = add(w[p], mul(z[p][c], v));
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) w[p] = mul(tau, w[p]);
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        const cx<R> v = ld<R>(g.vsh + c);
+#pragma unroll
+        for (int p = 0; p < P; ++p) z[p][c] = sub(z[p][c], mulc(w[p], v));
+      }
     }
     wave_sync();
     // ---- H <- (I - conj(tau) v v^H) H on rows i+1..n-1, columns i+1..n-1 ----
-    if (live && in && r >= i + 1) {
-      cx<R> y = mk<R>(0, 0);
-      for (int k = i + 1; k < n; ++k) y = add(y, cmul(ld<R>(g.vsh + k), ld<R>(g.at(k, r))));
-      const cx<R> ty = cmul(tau, y);  // conj(tau) * y
-      for (int k = i + 1; k < n; ++k)
-        st<R>(g.at(k, r), sub(ld<R>(g.at(k, r)), mul(ld<R>(g.vsh + k), ty)));
+    if (live) {
+      cx<R> y[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) y[p] = mk<R>(0, 0);
+      for (int k = i + 1; k < nmax; ++k) {
+        const cx<R> v = ld<R>(g.vsh + k);
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+          if ((p + 1) * LG > i + 1) y[p] = add(y[p], cmul(v, ld<R>(g.at(k, g.idx(p)))));
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) y[p] = cmul(tau, y[p]);  // conj(tau) * y
+      for (int k = i + 1; k < nmax; ++k) {
+        const cx<R> v = ld<R>(g.vsh + k);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const int col = g.idx(p);
+          if ((p + 1) * LG > i + 1 && col < n && col >= i + 1 && k < n)
+            st<R>(g.at(k, col), sub(ld<R>(g.at(k, col)), mul(v, y[p])));
+        }
+      }
     }
     wave_sync();
   }
@@ -292,20 +348,110 @@ __device__ __forceinline__ void larfg2(cx<R> a0, cx<R> a1, cx<R>& t1, cx<R>& v2,
   beta_out = mk<R>(-beta, 0);
 }
 
+// Apply one bulge-chase reflector at row K to this lane's slots: rows K, K+1
+// of H over the lane's columns >= K, columns K, K+1 of H over the lane's rows
+// <= i3, and columns K, K+1 of the Schur vectors.  KC >= 0 is K as a
+// compile-time constant (unrolled sweeps); KC < 0 takes the wavefront-uniform
+// runtime Krt and selects the register columns by a uniform branch tree.
+template <int KC, typename R, int NB, int P>
+__device__ __forceinline__ void sweep_step(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], bool act,
+                                           int L, int I, cx<R> a0, cx<R> a1, int Krt = 0) {
+  using L2 = typename Group<R, NB, P>::L2;
+  constexpr int LG = Group<R, NB, P>::LG;
+  const int n = g.n;
+  const int K = KC >= 0 ? KC : Krt;
+  cx<R> b0 = a0, b1 = a1;
+  if (act && K > L) {
+    b0 = ld<R>(g.at(K, K - 1));
+    b1 = ld<R>(g.at(K + 1, K - 1));
+  }
+  cx<R> t1 = mk<R>(0, 0), v2 = mk<R>(0, 0), nb0 = b0;
+  if (act) larfg2(b0, b1, t1, v2, nb0);
+  const R t2 = t1.re * v2.re - t1.im * v2.im;  // Re(t1 * v2)
+  if (act && K > L && g.r == 0) {
+    st<R>(g.at(K, K - 1), nb0);
+    st<R>(g.at(K + 1, K - 1), mk<R>(0, 0));
+  }
+  // rows K, K+1 over columns K..n-1 (slot = column)
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    if ((p + 1) * LG - 1 >= K) {
+      const int col = g.idx(p);
+      if (act && col >= K && col < n) {
+        L2* p1 = g.at(K, col);
+        L2* p2 = g.at(K + 1, col);
+        cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+        const cx<R> sum = add(cmul(t1, h1), scale(h2, t2));
+        h1 = sub(h1, sum);
+        h2 = sub(h2, mul(sum, v2));
+        st<R>(p1, h1);
+        st<R>(p2, h2);
+      }
+    }
+  }
+  wave_sync();
+  // columns K, K+1 over rows 0..min(K+2, I) (slot = row)
+  const int i3 = (K + 2) < I ? (K + 2) : I;
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    if (p * LG <= K + 2) {
+      const int row = g.idx(p);
+      if (act && row <= i3) {
+        L2* p1 = g.at(row, K);
+        L2* p2 = g.at(row, K + 1);
+        cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+        const cx<R> sum = add(mul(t1, h1), scale(h2, t2));
+        h1 = sub(h1, sum);
+        h2 = sub(h2, mulc(sum, v2));
+        st<R>(p1, h1);
+        st<R>(p2, h2);
+      }
+    }
+  }
+  // Schur vectors: columns K, K+1 of each owned row
+  if (act) {
+    auto upd = [&](auto kc) {
+      constexpr int KK = decltype(kc)::value;
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const cx<R> sum = add(mul(t1, z[p][KK]), scale(z[p][KK + 1], t2));
+        z[p][KK] = sub(z[p][KK], sum);
+        z[p][KK + 1] = sub(z[p][KK + 1], mulc(sum, v2));
+      }
+    };
+    if constexpr (KC >= 0) upd(std::integral_constant<int, KC>{});
+    else uniform_dispatch<0, NB - 1>(K, upd);
+  }
+  wave_sync();
+}
+
+template <int K, int KEND, typename R, int NB, int P>
+__device__ __forceinline__ void sweep_range(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], bool ready,
+                                            int L, int I, cx<R> a0, cx<R> a1, int kmin, int kmax) {
+  if constexpr (K < KEND) {
+    if (K >= kmin && K < kmax) sweep_step<K>(g, z, ready && K >= L && K < I, L, I, a0, a1);
+    sweep_range<K + 1, KEND>(g, z, ready, L, I, a0, a1, kmin, kmax);
+  }
+}
+
 // --------------------------------------------------------------------------
 // Single-shift complex QR to Schur form with Schur vectors (zlahqr_schur with
-// wantT = wantZ = 1, Toolbox.hpp:690-1031).  Deflation test (Ahues-Tisseur),
-// Wilkinson shift and exceptional shifts at iterations 10 and 20 as in the
-// reference; itmax sweeps per eigenvalue before it is accepted anyway (30 in
-// the reference, Toolbox.hpp:808); the count of such eigenvalues is returned.
+// wantT = wantZ = 1, Toolbox.hpp:690-1031): the reference's deflation test
+// (Ahues & Tisseur), its exceptional shifts at iterations 10 and 20 and its
+// Wilkinson shift otherwise.  Every matrix of the wavefront iterates
+// independently; a sweep is one pass of the bulge chase over the union of
+// the active row ranges.  An eigenvalue not converged after itmax + 1 sweeps
+// is accepted as it stands (the reference's silent case, Toolbox.hpp:808);
+// the count of such eigenvalues is returned.
 // If make_real_subdiag, the subdiagonal is first made real (loop 20,
 // Toolbox.hpp:735-793); after gehrd_q it already is.
 // --------------------------------------------------------------------------
-template <typename R, int NB>
-__device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int itmax,
+template <typename R, int NB, int P>
+__device__ __forceinline__ int lahqr(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], int itmax,
                                      bool make_real_subdiag) {
-  using L2 = typename Group<R, NB>::L2;
-  const int r = g.r, n = g.n;
+  constexpr int LG = Group<R, NB, P>::LG;
+  constexpr int MPW = Group<R, NB, P>::MPW;
+  const int n = g.n;
   const R ulp = limits<R>::eps;
   const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
 
@@ -322,10 +468,19 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
       }
       wave_sync();
       if (doit) {
-        if (r == i) st<R>(g.at(i, i - 1), mk<R>(fsqrt(abs2(h)), 0));
-        if (r >= i && r < n) st<R>(g.at(i, r), mul(ld<R>(g.at(i, r)), sc));
-        if (r <= i + 1 && r < n) st<R>(g.at(r, i), mulc(ld<R>(g.at(r, i)), sc));
-        z[i] = mulc(z[i], sc);
+        if (g.r == 0) st<R>(g.at(i, i - 1), mk<R>(fsqrt(abs2(h)), 0));
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const int x = g.idx(p);
+          if (x >= i && x < n) st<R>(g.at(i, x), mul(ld<R>(g.at(i, x)), sc));
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const int x = g.idx(p);
+          if (x <= i + 1 && x < n) st<R>(g.at(x, i), mulc(ld<R>(g.at(x, i)), sc));
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) z[p][i] = mulc(z[p][i], sc);
       }
       wave_sync();
     }
@@ -333,17 +488,20 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
 
   int I = n - 1, L = 0, its = 0, nfail = 0;
   bool done = (n <= 0);
-  const int base = g.mi * NB;
+  const int base = g.mi * LG;
+  constexpr uint64_t kMask = (LG == 64) ? ~0ull : ((1ull << LG) - 1);
   for (;;) {
     // ---- bookkeeping until every matrix either needs a sweep or is done ----
     bool ready = false;
     for (;;) {
       const bool need = !done && !ready;
       if (!__any(need)) break;
-      bool neg = false;
-      if (need && its <= itmax) {
-        const int k = r;
-        if (k > L && k <= I) {
+      uint64_t bal[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        bool neg = false;
+        const int k = g.idx(p);
+        if (need && its <= itmax && k > L && k <= I) {
           const cx<R> h = ld<R>(g.at(k, k - 1));
           if (cabs1(h) <= smlnum) {
             neg = true;
@@ -364,8 +522,8 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
             }
           }
         }
+        bal[p] = __ballot(neg);
       }
-      const uint64_t bal = __ballot(neg);
       if (need) {
         if (its > itmax) {
           // accepted without convergence (the reference's silent case)
@@ -375,10 +533,14 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
           its = 0;
           if (I < 0) done = true;
         } else {
-          // this group's NB ballot bits (NB = 64 uses all 64)
-          const uint64_t bits = (bal >> base) & ((NB == 64) ? ~0ull : ((1ull << NB) - 1));
-          if (bits) L = 63 - __clzll(bits);
-          if (L > 0 && r == L) st<R>(g.at(L, L - 1), mk<R>(0, 0));
+          // the largest k with a negligible H(k, k-1) (the reference's scan
+          // from I down to L+1); L is unchanged if there is none
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const uint64_t bits = (bal[p] >> base) & kMask;
+            if (bits) L = p * LG + 63 - __clzll(bits);
+          }
+          if (L > 0 && g.r == 0) st<R>(g.at(L, L - 1), mk<R>(0, 0));
           if (L >= I) {
             --I;
             L = 0;
@@ -433,10 +595,10 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
     // ---- wavefront-uniform range of bulge positions ----
     int kmin = NB, kmax = 0;
 #pragma unroll
-    for (int q = 0; q < Group<R, NB>::MPW; ++q) {
-      const int rq = __builtin_amdgcn_readlane(ready ? 1 : 0, q * NB);
-      const int lq = __builtin_amdgcn_readlane(L, q * NB);
-      const int iq = __builtin_amdgcn_readlane(I, q * NB);
+    for (int q = 0; q < MPW; ++q) {
+      const int rq = __builtin_amdgcn_readlane(ready ? 1 : 0, q * LG);
+      const int lq = __builtin_amdgcn_readlane(L, q * LG);
+      const int iq = __builtin_amdgcn_readlane(I, q * LG);
       if (rq) {
         kmin = lq < kmin ? lq : kmin;
         kmax = iq > kmax ? iq : kmax;
@@ -444,106 +606,13 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
     }
 
     // ---- the sweep (Toolbox.hpp:138-335), K = bulge row ----
-    if constexpr (Group<R, NB>::MPW == 1) {
-      for (int K = kmin; K < kmax; ++K) {
-        const bool act = ready && K >= L && K < I;
-        cx<R> b0 = a0, b1 = a1;
-        if (act && K > L) {
-          b0 = ld<R>(g.at(K, K - 1));
-          b1 = ld<R>(g.at(K + 1, K - 1));
-        }
-        cx<R> t1 = mk<R>(0, 0), v2 = mk<R>(0, 0), nb0 = b0;
-        if (act) larfg2(b0, b1, t1, v2, nb0);
-        const R t2 = t1.re * v2.re - t1.im * v2.im;  // Re(t1 * v2)
-        if (act && K > L) {
-          if (r == K) st<R>(g.at(K, K - 1), nb0);
-          if (r == K + 1) st<R>(g.at(K + 1, K - 1), mk<R>(0, 0));
-        }
-        // rows K, K+1 over columns K..n-1 (lane = column)
-        if (act && r >= K && r < n) {
-          L2* p1 = g.at(K, r);
-          L2* p2 = g.at(K + 1, r);
-          cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
-          const cx<R> sum = add(cmul(t1, h1), scale(h2, t2));
-          h1 = sub(h1, sum);
-          h2 = sub(h2, mul(sum, v2));
-          st<R>(p1, h1);
-          st<R>(p2, h2);
-        }
-        wave_sync();
-        // columns K, K+1 over rows 0..min(K+2, I) (lane = row)
-        const int i3 = (K + 2) < I ? (K + 2) : I;
-        if (act && r <= i3) {
-          L2* p1 = g.at(r, K);
-          L2* p2 = g.at(r, K + 1);
-          cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
-          const cx<R> sum = add(mul(t1, h1), scale(h2, t2));
-          h1 = sub(h1, sum);
-          h2 = sub(h2, mulc(sum, v2));
-          st<R>(p1, h1);
-          st<R>(p2, h2);
-        }
-        // Schur vectors: columns K, K+1 of row r (K is wavefront-uniform)
-        if (act) {
-          uniform_dispatch<0, NB - 1>(K, [&](auto kc) {
-            constexpr int KK = decltype(kc)::value;
-            const cx<R> sum = add(mul(t1, z[KK]), scale(z[KK + 1], t2));
-            z[KK] = sub(z[KK], sum);
-            z[KK + 1] = sub(z[KK + 1], mulc(sum, v2));
-          });
-        }
-        wave_sync();
-      }
+    if constexpr (MPW == 1) {
+      // one matrix per wavefront: a runtime loop over K, with the Schur
+      // vector columns selected by a uniform branch tree
+      static_assert(P == 1, "one matrix per wavefront uses one index per lane");
+      for (int K = kmin; K < kmax; ++K) sweep_step<-1>(g, z, ready && K >= L && K < I, L, I, a0, a1, K);
     } else {
-#pragma unroll
-    for (int K = 0; K < NB - 1; ++K) {
-      if (K >= kmin && K < kmax) {
-        const bool act = ready && K >= L && K < I;
-        cx<R> b0 = a0, b1 = a1;
-        if (act && K > L) {
-          b0 = ld<R>(g.at(K, K - 1));
-          b1 = ld<R>(g.at(K + 1, K - 1));
-        }
-        cx<R> t1 = mk<R>(0, 0), v2 = mk<R>(0, 0), nb0 = b0;
-        if (act) larfg2(b0, b1, t1, v2, nb0);
-        const R t2 = t1.re * v2.re - t1.im * v2.im;  // Re(t1 * v2)
-        if (act && K > L) {
-          if (r == K) st<R>(g.at(K, K - 1), nb0);
-          if (r == K + 1) st<R>(g.at(K + 1, K - 1), mk<R>(0, 0));
-        }
-        // rows K, K+1 over columns K..n-1 (lane = column)
-        if (act && r >= K && r < n) {
-          L2* p1 = g.at(K, r);
-          L2* p2 = g.at(K + 1, r);
-          cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
-          const cx<R> sum = add(cmul(t1, h1), scale(h2, t2));
-          h1 = sub(h1, sum);
-          h2 = sub(h2, mul(sum, v2));
-          st<R>(p1, h1);
-          st<R>(p2, h2);
-        }
-        wave_sync();
-        // columns K, K+1 over rows 0..min(K+2, I) (lane = row)
-        const int i3 = (K + 2) < I ? (K + 2) : I;
-        if (act && r <= i3) {
-          L2* p1 = g.at(r, K);
-          L2* p2 = g.at(r, K + 1);
-          cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
-          const cx<R> sum = add(mul(t1, h1), scale(h2, t2));
-          h1 = sub(h1, sum);
-          h2 = sub(h2, mulc(sum, v2));
-          st<R>(p1, h1);
-          st<R>(p2, h2);
-        }
-        // Schur vectors: columns K, K+1 of row r
-        if (act) {
-          const cx<R> sum = add(mul(t1, z[K]), scale(z[K + 1], t2));
-          z[K] = sub(z[K], sum);
-          z[K + 1] = sub(z[K + 1], mulc(sum, v2));
-        }
-        wave_sync();
-      }
-    }
+      sweep_range<0, NB - 1>(g, z, ready, L, I, a0, a1, kmin, kmax);
     }
 
     // ---- make H(I, I-1) real (Toolbox.hpp:337-367) ----
@@ -553,18 +622,25 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
         const R id = frsq(abs2(h));
         const R d = abs2(h) * id;
         const cx<R> zz = mk(h.re * id, -h.im * id);  // conj(h/|h|)
-        if (r == I) st<R>(g.at(I, I - 1), mk<R>(d, 0));
-        if (r > I && r < n) st<R>(g.at(I, r), mul(ld<R>(g.at(I, r)), zz));
-        if (r < I) st<R>(g.at(r, I), mulc(ld<R>(g.at(r, I)), zz));
-        if constexpr (Group<R, NB>::MPW == 1) {
+        if (g.r == 0) st<R>(g.at(I, I - 1), mk<R>(d, 0));
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const int x = g.idx(p);
+          if (x > I && x < n) st<R>(g.at(I, x), mul(ld<R>(g.at(I, x)), zz));
+          if (x < I) st<R>(g.at(x, I), mulc(ld<R>(g.at(x, I)), zz));
+        }
+        if constexpr (MPW == 1) {
           uniform_dispatch<0, NB>(I, [&](auto kc) {
             constexpr int KK = decltype(kc)::value;
-            z[KK] = mulc(z[KK], zz);
+            z[0][KK] = mulc(z[0][KK], zz);
           });
         } else {
 #pragma unroll
           for (int c = 1; c < NB; ++c)
-            if (c == I) z[c] = mulc(z[c], zz);
+            if (c == I) {
+#pragma unroll
+              for (int p = 0; p < P; ++p) z[p][c] = mulc(z[p][c], zz);
+            }
         }
       }
       ++its;
@@ -576,119 +652,148 @@ __device__ __forceinline__ int lahqr(const Group<R, NB>& g, cx<R> (&z)[NB], int 
 
 // --------------------------------------------------------------------------
 // Right eigenvectors of the upper-triangular T in LDS (ztrevc +
-// backsubstitution, Toolbox.hpp:1133-1313, 31-136).  Lane k solves
-// (T(0:k-1,0:k-1) - T(k,k) I) x = -T(0:k-1,k) with the reference's
+// backsubstitution, Toolbox.hpp:1133-1313, 31-136).  The lane owning index k
+// solves (T(0:k-1,0:k-1) - T(k,k) I) x = -T(0:k-1,k) with the reference's
 // small-pivot clamp (Toolbox.hpp:1235-1258); the unit upper triangular X
 // (column k = [x; 1; 0]) is written over the strict upper triangle of T.
-// Returns lam = T(r,r) in lane r.
+// Slots are solved from the last to the first: slot p only reads columns
+// below (p+1) LG, which later slots have not yet overwritten.
+// Returns lam[p] = T(k,k) for k = idx(p).
 // --------------------------------------------------------------------------
-template <typename R, int NB>
-__device__ __forceinline__ cx<R> trevc_solve(const Group<R, NB>& g, int nmax) {
-  const int r = g.r, n = g.n;
+template <typename R, int NB, int P>
+__device__ __forceinline__ void trevc_solve(const Group<R, NB, P>& g, int nmax, cx<R> (&lam)[P]) {
+  constexpr int LG = Group<R, NB, P>::LG;
+  const int n = g.n;
   const R ulp = limits<R>::eps;
   const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
-  const bool in = r < n;
-  const cx<R> lam = in ? ld<R>(g.at(r, r)) : mk<R>(0, 0);
-  const R smin = fmax(ulp * cabs1(lam), smlnum);
-  cx<R> x[NB];
 #pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    const bool a = in && j < r;
-    x[j] = a ? ld<R>(g.at(j, r)) : mk<R>(0, 0);
-    x[j] = mk(-x[j].re, -x[j].im);
-  }
+  for (int p = P - 1; p >= 0; --p) {
+    const int k = g.idx(p);
+    const int kcap = (p + 1) * LG;  // k < kcap in this slot
+    const bool in = k < n;
+    lam[p] = in ? ld<R>(g.at(k, k)) : mk<R>(0, 0);
+    const R smin = fmax(ulp * cabs1(lam[p]), smlnum);
+    cx<R> x[NB];
 #pragma unroll
-  for (int j = NB - 2; j >= 0; --j) {
-    if (j + 1 < nmax) {
-      const bool act = in && j < r;
-      cx<R> d = sub(ld<R>(g.at(j, j)), lam);
-      if (cabs1(d) < smin) d = mk<R>(smin, 0);
-      const cx<R> xj = mul(x[j], inv(d));
-      if (act) x[j] = xj;
+    for (int j = 0; j < NB; ++j) {
+      const bool a = j < kcap && in && j < k;
+      x[j] = a ? ld<R>(g.at(j, k)) : mk<R>(0, 0);
+      x[j] = mk(-x[j].re, -x[j].im);
+    }
 #pragma unroll
-      for (int i = 0; i < j; ++i) {
-        const cx<R> tij = ld<R>(g.at(i, j));
-        if (act) x[i] = sub(x[i], mul(tij, xj));
+    for (int j = NB - 2; j >= 0; --j) {
+      if (j + 1 < kcap && j + 1 < nmax) {
+        const bool act = in && j < k;
+        cx<R> d = sub(ld<R>(g.at(j, j)), lam[p]);
+        if (cabs1(d) < smin) d = mk<R>(smin, 0);
+        const cx<R> xj = mul(x[j], inv(d));
+        if (act) x[j] = xj;
+#pragma unroll
+        for (int i = 0; i < j; ++i) {
+          const cx<R> tij = ld<R>(g.at(i, j));
+          if (act) x[i] = sub(x[i], mul(tij, xj));
+        }
       }
     }
-  }
-  wave_sync();
+    wave_sync();
 #pragma unroll
-  for (int j = 0; j < NB; ++j)
-    if (in && j < r) st<R>(g.at(j, r), x[j]);
-  wave_sync();
-  return lam;
+    for (int j = 0; j < NB; ++j)
+      if (j < kcap && in && j < k) st<R>(g.at(j, k), x[j]);
+    wave_sync();
+  }
 }
 
 // --------------------------------------------------------------------------
 // Approximate inverse of the eigenvector matrix, W = X^{-1} Z^H, from the X
-// left in LDS by trevc_solve and the Schur-vector row z[] of each lane.
-// Lane k forms column k of Y = X^{-1} by back substitution and stores its
-// strict upper part transposed into the (unused) strict lower triangle of the
-// tile; then lane r forms column r of W, returned in w[] (w[k] = W(k, r)).
+// left in LDS by trevc_solve and the Schur-vector rows z[p][] of each lane.
+// The lane owning index k forms column k of Y = X^{-1} by back substitution
+// and stores its strict upper part transposed into the (unused) strict lower
+// triangle of the tile; then for each owned row r of Z it produces column r
+// of W, passed as out(p, k, W(k, r)).
 // --------------------------------------------------------------------------
-template <typename R, int NB>
-__device__ __forceinline__ void inverse_vectors(const Group<R, NB>& g, const cx<R> (&z)[NB],
-                                                cx<R> (&w)[NB], int nmax) {
-  const int r = g.r, n = g.n;
-  const bool in = r < n;
-  cx<R> y[NB];
+template <typename R, int NB, int P, typename Out>
+__device__ __forceinline__ void inverse_vectors(const Group<R, NB, P>& g, const cx<R> (&z)[P][NB],
+                                                int nmax, Out&& out) {
+  constexpr int LG = Group<R, NB, P>::LG;
+  const int n = g.n;
 #pragma unroll
-  for (int j = 0; j < NB; ++j) y[j] = mk<R>(j == r ? R(1) : R(0), 0);
-  // y_j = -sum_{i=j+1..k} X(j,i) y_i, j = k-1 .. 0
+  for (int p = 0; p < P; ++p) {
+    const int k = g.idx(p);
+    const int kcap = (p + 1) * LG;
+    const bool in = k < n;
+    cx<R> y[NB];
 #pragma unroll
-  for (int j = NB - 2; j >= 0; --j) {
-    if (j + 1 < nmax) {
-      cx<R> acc = mk<R>(0, 0);
+    for (int j = 0; j < NB; ++j) y[j] = mk<R>(j == k ? R(1) : R(0), 0);
+    // y_j = -sum_{i=j+1..k} X(j,i) y_i, j = k-1 .. 0
 #pragma unroll
-      for (int i = j + 1; i < NB; ++i) acc = add(acc, mul(ld<R>(g.at(j, i)), y[i]));
-      if (in && j < r) y[j] = mk(-acc.re, -acc.im);
+    for (int j = NB - 2; j >= 0; --j) {
+      if (j + 1 < kcap && j + 1 < nmax) {
+        cx<R> acc = mk<R>(0, 0);
+#pragma unroll
+        for (int i = j + 1; i < NB; ++i)
+          if (i < kcap) acc = add(acc, mul(ld<R>(g.at(j, i)), y[i]));
+        if (in && j < k) y[j] = mk(-acc.re, -acc.im);
+      }
     }
-  }
-  // Y(j,k) (j < k) -> tile position (k, j)
+    // Y(j,k) (j < k) -> tile position (k, j)
 #pragma unroll
-  for (int j = 0; j < NB; ++j)
-    if (in && j < r) st<R>(g.at(r, j), y[j]);
+    for (int j = 0; j < NB; ++j)
+      if (j < kcap && in && j < k) st<R>(g.at(k, j), y[j]);
+  }
   wave_sync();
   // W(k, r) = conj(z[k]) + sum_{j>k} Y(k,j) conj(z[j])
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
-    cx<R> acc = conj(z[k]);
+    cx<R> acc[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) acc[p] = conj(z[p][k]);
     if (k < nmax) {
 #pragma unroll
-      for (int j = k + 1; j < NB; ++j) acc = add(acc, mulc(ld<R>(g.at(j, k)), z[j]));
+      for (int j = k + 1; j < NB; ++j) {
+        const cx<R> yk = ld<R>(g.at(j, k));
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] = add(acc[p], mulc(yk, z[p][j]));
+      }
     }
-    w[k] = acc;
+#pragma unroll
+    for (int p = 0; p < P; ++p) out(p, k, acc[p]);
   }
 }
 
 // --------------------------------------------------------------------------
-// Back-transform (the zgemv of ztrevc, Toolbox.hpp:1272-1275): row r of
-// V = Z * X, in place in z[] (column k of V = Z [x; 1; 0], the reference's
-// scaling, not normalised).
+// Back-transform (the zgemv of ztrevc, Toolbox.hpp:1272-1275): rows idx(p)
+// of V = Z * X, in place in z[][] (column k of V = Z [x; 1; 0], the
+// reference's scaling, not normalised).
 // --------------------------------------------------------------------------
-template <typename R, int NB>
-__device__ __forceinline__ void backtransform(const Group<R, NB>& g, cx<R> (&z)[NB], int nmax) {
+template <typename R, int NB, int P>
+__device__ __forceinline__ void backtransform(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], int nmax) {
 #pragma unroll
   for (int k = NB - 1; k >= 1; --k) {
     if (k < nmax) {
-      cx<R> acc = z[k];
+      cx<R> acc[P];
 #pragma unroll
-      for (int j = 0; j < k; ++j) acc = add(acc, mul(z[j], ld<R>(g.at(j, k))));
-      z[k] = acc;
+      for (int p = 0; p < P; ++p) acc[p] = z[p][k];
+#pragma unroll
+      for (int j = 0; j < k; ++j) {
+        const cx<R> xjk = ld<R>(g.at(j, k));
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] = add(acc[p], mul(z[p][j], xjk));
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) z[p][k] = acc[p];
     }
   }
 }
 
 // --------------------------------------------------------------------------
-// The same eigenvector solve for one matrix per wavefront (n up to 64), with
-// the right-hand sides kept in LDS instead of registers: lane k keeps
-// y = -x of its eigenvector in column k of the tile (rows < k).  At step j
-// lanes k > j read column j of T, which lane j only starts to overwrite at
-// step j-1, so T and the solutions share the tile.
+// The same eigenvector solve for one matrix per wavefront (n up to 64, one
+// index per lane), with the right-hand sides kept in LDS instead of
+// registers: lane k keeps y = -x of its eigenvector in column k of the tile
+// (rows < k).  At step j lanes k > j read column j of T, which lane j only
+// starts to overwrite at step j-1, so T and the solutions share the tile.
 // --------------------------------------------------------------------------
 template <typename R, int NB>
-__device__ __forceinline__ cx<R> trevc_solve_lds(const Group<R, NB>& g, int nmax) {
+__device__ __forceinline__ cx<R> trevc_solve_lds(const Group<R, NB, 1>& g, int nmax) {
   const int r = g.r, n = g.n;
   const R ulp = limits<R>::eps;
   const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
@@ -716,7 +821,7 @@ __device__ __forceinline__ cx<R> trevc_solve_lds(const Group<R, NB>& g, int nmax
 // Row r of V = Z X from the tile left by trevc_solve_lds, one column at a
 // time, stored straight to the output (column k of V = Z [x; 1; 0]).
 template <typename R, int NB, typename Store>
-__device__ __forceinline__ void backtransform_lds(const Group<R, NB>& g, const cx<R> (&z)[NB],
+__device__ __forceinline__ void backtransform_lds(const Group<R, NB, 1>& g, const cx<R> (&z)[1][NB],
                                                   int nmax, Store&& store) {
   const int r = g.r;
   // tile column r becomes -X(:, r): y above the diagonal, -1 on it, 0 below
@@ -729,16 +834,16 @@ __device__ __forceinline__ void backtransform_lds(const Group<R, NB>& g, const c
     cx<R> acc = mk<R>(0, 0);
 #pragma unroll
     for (int j = 0; j < NB; ++j)
-      if (j < nmax) acc = add(acc, mul(z[j], ld<R>(g.at(j, k))));
+      if (j < nmax) acc = add(acc, mul(z[0][j], ld<R>(g.at(j, k))));
     store(k, mk(-acc.re, -acc.im));
   }
 }
 
-template <typename R, int NB>
-__device__ __forceinline__ void trevc_backtransform(const Group<R, NB>& g, cx<R> (&z)[NB],
-                                                    cx<R>& lam, int nmax) {
-  lam = trevc_solve<R, NB>(g, nmax);
-  backtransform<R, NB>(g, z, nmax);
+template <typename R, int NB, int P>
+__device__ __forceinline__ void trevc_backtransform(const Group<R, NB, P>& g, cx<R> (&z)[P][NB],
+                                                    cx<R> (&lam)[P], int nmax) {
+  trevc_solve<R, NB, P>(g, nmax, lam);
+  backtransform<R, NB, P>(g, z, nmax);
 }
 
 }  // namespace veig

@@ -15,7 +15,7 @@
 //                  per-order choice of `nb_of_n`), ordered by n inside a group:
 //                  start offsets and the three bin ranges
 //   k_bin_scatter  permutation of matrix indices in that order
-//   k_zgeev_var<NB> one persistent kernel per bin: a fixed grid walks the bin's
+//   k_zgeev_var<NB,P> one persistent kernel per bin: a fixed grid walks the bin's
 //                  range, read from device memory, wavefront by wavefront; the
 //                  matrices of a wavefront then have (nearly) the same order.
 #include <hip/hip_runtime.h>
@@ -108,25 +108,26 @@ __global__ void k_bin_scatter(VarArgs a) {
   }
 }
 
-template <int NB>
+template <int NB, int P>
 __global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
   using R = double;
-  __shared__ Tiles<R, NB> sm;
-  constexpr int MPW = Tiles<R, NB>::MPW;
+  using Gr = Group<R, NB, P>;
+  __shared__ Tiles<R, NB, P> sm;
+  constexpr int MPW = Gr::MPW;
   const int64_t start = a.bins->start[bin], count = a.bins->count[bin];
   const int64_t nwaves = (count + MPW - 1) / MPW;
   for (int64_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
-    const int64_t idx = w * MPW + threadIdx.x / NB;
+    const int64_t idx = w * MPW + threadIdx.x / Gr::LG;
     const bool valid = idx < count;
     const int64_t m = valid ? a.perm[start + idx] : 0;
     const int n = valid ? a.dims[m] : 0;
     int nmax = 0;
 #pragma unroll
     for (int q = 0; q < MPW; ++q) {
-      const int nq = __builtin_amdgcn_readlane(n, q * NB);
+      const int nq = __builtin_amdgcn_readlane(n, q * Gr::LG);
       nmax = nq > nmax ? nq : nmax;
     }
-    Group<R, NB> g = make_group<R, NB>(sm, n);
+    Gr g = make_group<R, NB, P>(sm, n);
     Geom G;
     G.nmat = 0;
     G.T = 1;
@@ -134,19 +135,17 @@ __global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
     const int64_t base = valid ? a.a_off[m] : 0;
     const int64_t vbase = valid ? a.e_off[m] : 0;
     wave_sync();
-    load_rows<R, NB>(g, G, base, a.rA, a.iA, NB);
-    cx<R> z[NB];
-    identity_rows<R, NB>(g, z);
+    load_rows<R, NB, P>(g, G, base, a.rA, a.iA, NB);
+    cx<R> z[P][NB];
+    identity_rows<R, NB, P>(g, z);
     wave_sync();
-    gehrd_q<R, NB>(g, z, nmax, true, false, nullptr);
-    const int nfail = lahqr<R, NB>(g, z, a.itmax, false);
-    const cx<R> lam = trevc_solve<R, NB>(g, nmax);
-    backtransform<R, NB>(g, z, nmax);
-    store_row_regs<R, NB>(g, G, base, a.rV, a.iV, z);
-    if (g.r < g.n) {
-      a.rE[vbase + g.r] = lam.re;
-      a.iE[vbase + g.r] = lam.im;
-    }
+    gehrd_q<R, NB, P>(g, z, nmax, true, false, nullptr);
+    const int nfail = lahqr<R, NB, P>(g, z, a.itmax, false);
+    cx<R> lam[P];
+    trevc_solve<R, NB, P>(g, nmax, lam);
+    backtransform<R, NB, P>(g, z, nmax);
+    store_row_regs<R, NB, P>(g, G, base, a.rV, a.iV, z);
+    store_eigenvalues<R, NB, P>(g, G, vbase, a.rE, a.iE, lam);
     if (a.info && valid && g.r == 0) a.info[m] = nfail;
     wave_sync();
   }
@@ -194,9 +193,15 @@ int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, con
   for (int b = 0; b < kBins; ++b) {
     const int g = (grid_per_bin && grid_per_bin[b] > 0) ? grid_per_bin[b] : defaults[b];
     switch (b) {
-      case 0: hipLaunchKernelGGL((k_zgeev_var<8>), dim3(g), dim3(64), 0, s, a, 0); break;
-      case 1: hipLaunchKernelGGL((k_zgeev_var<16>), dim3(g), dim3(64), 0, s, a, 1); break;
-      default: hipLaunchKernelGGL((k_zgeev_var<32>), dim3(g), dim3(64), 0, s, a, 2); break;
+      case 0:
+        if (slots_for(false, 8) == 2) hipLaunchKernelGGL((k_zgeev_var<8, 2>), dim3(g), dim3(64), 0, s, a, 0);
+        else hipLaunchKernelGGL((k_zgeev_var<8, 1>), dim3(g), dim3(64), 0, s, a, 0);
+        break;
+      case 1:
+        if (slots_for(false, 16) == 2) hipLaunchKernelGGL((k_zgeev_var<16, 2>), dim3(g), dim3(64), 0, s, a, 1);
+        else hipLaunchKernelGGL((k_zgeev_var<16, 1>), dim3(g), dim3(64), 0, s, a, 1);
+        break;
+      default: hipLaunchKernelGGL((k_zgeev_var<32, 1>), dim3(g), dim3(64), 0, s, a, 2); break;
     }
   }
   VEIG_HIP(hipGetLastError());
