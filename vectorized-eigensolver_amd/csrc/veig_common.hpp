@@ -212,17 +212,38 @@ struct FusedArgs {
   int itmax;
   uint32_t flags;
   const int32_t *mask;  // if set, only matrices with mask[m] != 0 are solved
-  float *rW, *iW;       // if set (WANT_W), W = V^{-1} in complex64 planes
+  float *rW, *iW;       // OUT_VW: W = V^{-1}; OUT_ZXY: the Schur vectors Z (complex64 planes)
+  float *rX = nullptr, *iX = nullptr;  // OUT_ZXY: X strictly above, X^{-1} strictly below the diagonal
+  // if set, solve only the matrices list[0 .. *list_count) (device memory),
+  // with a persistent grid walking the list
+  const int32_t *list = nullptr;
+  const int32_t *list_count = nullptr;
 };
 
-template <typename R, int NB, int P, bool WANT_W>
-__global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
-  __shared__ Tiles<R, NB, P> sm;
+// What the fused kernel writes: the eigenvectors V; V and an approximate
+// inverse W = V^{-1} (complex64); or, for refinement on the matrix cores, the
+// Schur vectors Z and the eigenvector factor X of T (V = Z X) with
+// Y = X^{-1} (complex64), instead of V.
+enum { OUT_V = 0, OUT_VW = 1, OUT_ZXY = 2 };
+
+// one wavefront of the fused kernel: MPW matrices, from the batch (wave
+// index `wave`) or from a.list (list entries wave*MPW ..)
+template <typename R, int NB, int P, int OUT>
+__device__ __forceinline__ void zgeev_wave(const FusedArgs &a, Tiles<R, NB, P> &sm, int64_t wave,
+                                           int64_t list_count) {
   using G = Group<R, NB, P>;
   constexpr int MPW = G::MPW;
-  const int64_t m = xcd_block() * MPW + threadIdx.x / G::LG;
-  bool valid = m < a.G.nmat;
-  if (valid && a.mask) valid = a.mask[m] != 0;
+  const int64_t slot = wave * MPW + threadIdx.x / G::LG;
+  int64_t m;
+  bool valid;
+  if (a.list) {
+    valid = slot < list_count;
+    m = valid ? a.list[slot] : 0;
+  } else {
+    m = slot;
+    valid = m < a.G.nmat;
+    if (valid && a.mask) valid = a.mask[m] != 0;
+  }
   if (!__any(valid)) return;  // wave-uniform early exit (masked launches)
   const int n = valid ? a.G.n : 0;
   G g = make_group<R, NB, P>(sm, n);
@@ -239,7 +260,7 @@ __global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
   if (a.flags & VEIG_WRITE_SCHUR) store_rows<R, NB, P>(g, a.G, base, a.rA, a.iA);
   if constexpr (NB == 64) {
     // one matrix per wavefront: eigenvector solves in LDS, V streamed out
-    static_assert(!WANT_W && P == 1, "one matrix per wavefront: FP64, one index per lane");
+    static_assert(OUT == OUT_V && P == 1, "one matrix per wavefront: FP64, one index per lane");
     const cx<R> lam = trevc_solve_lds<R, NB>(g, nmax);
     backtransform_lds<R, NB>(g, z, nmax, [&](int k, cx<R> v) {
       if (g.r < g.n) {
@@ -257,7 +278,40 @@ __global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
   }
   cx<R> lam[P];
   trevc_solve<R, NB, P>(g, nmax, lam);
-  if constexpr (WANT_W) {
+  if constexpr (OUT == OUT_ZXY) {
+    // Z first, so that its registers are free for the triangular inverse
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int row = g.idx(p);
+      if (row < g.n) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          if (c < g.n) {
+            const int64_t idx = base + a.G.el(row, c);
+            a.rW[idx] = (float)z[p][c].re;
+            a.iW[idx] = (float)z[p][c].im;
+          }
+        }
+      }
+    }
+    inverse_unit_upper<R, NB, P>(g, nmax);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int row = g.idx(p);
+      if (row < g.n) {
+        for (int c = 0; c < g.n; ++c) {
+          const int64_t idx = base + a.G.el(row, c);
+          const cx<R> t = ld<R>(g.at(row, c));
+          a.rX[idx] = (float)t.re;
+          a.iX[idx] = (float)t.im;
+        }
+      }
+    }
+    store_eigenvalues<R, NB, P>(g, a.G, vbase, a.rE, a.iE, lam);
+    if (a.info && valid && g.r == 0) a.info[m] = nfail;
+    return;
+  }
+  if constexpr (OUT == OUT_VW) {
     // column idx(p) of W
     inverse_vectors<R, NB, P>(g, z, nmax, [&](int p, int k, cx<R> w) {
       const int col = g.idx(p);
@@ -274,19 +328,58 @@ __global__ __launch_bounds__(64) void k_zgeev(FusedArgs a) {
   if (a.info && valid && g.r == 0) a.info[m] = nfail;
 }
 
+// the batch, one wavefront per MPW matrices (kept free of the list loop:
+// a second inlined copy of the solver would double the code footprint)
+// occupancy the register allocator is asked to keep (1: no constraint); the
+// complex64 kernel that feeds the matrix-core refinement fits 4 wavefronts
+template <typename R, int NB, int P, int OUT>
+constexpr int zgeev_waves_per_eu() {
+  return (std::is_same<R, float>::value && OUT == OUT_ZXY && NB == 16 && P == 1) ? 4 : 1;
+}
+
+template <typename R, int NB, int P, int OUT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(zgeev_waves_per_eu<R, NB, P, OUT>())))
+void k_zgeev(FusedArgs a) {
+  __shared__ Tiles<R, NB, P> sm;
+  zgeev_wave<R, NB, P, OUT>(a, sm, xcd_block(), 0);
+}
+
+// the matrices of a.list, on a persistent grid
+template <typename R, int NB, int P, int OUT>
+__global__ __launch_bounds__(64) void k_zgeev_list(FusedArgs a) {
+  __shared__ Tiles<R, NB, P> sm;
+  constexpr int MPW = Group<R, NB, P>::MPW;
+  const int64_t cnt = *a.list_count;
+  const int64_t nw = (cnt + MPW - 1) / MPW;
+  for (int64_t w = blockIdx.x; w < nw; w += gridDim.x) {
+    zgeev_wave<R, NB, P, OUT>(a, sm, w, cnt);
+    wave_sync();
+  }
+}
+
 // Lane slots per index of the fused kernel, per precision and padded order
 // (P = 1: one lane per index).  VEIG_SLOTS=<p> overrides the table where the
 // kernel exists with that many slots (tuning).
 int slots_for(bool fp32, int nb);
 
-template <typename R, int NB, int P, bool WANT_W>
+// persistent grid of list launches (the list length is only known on the device)
+constexpr int64_t kListGrid = 4096;
+
+template <typename R, int NB, int P, int OUT>
 inline void launch_one(const FusedArgs &a, hipStream_t s) {
   constexpr int mpw = Group<R, NB, P>::MPW;
   const int64_t grid = (a.G.nmat + mpw - 1) / mpw;
-  hipLaunchKernelGGL((k_zgeev<R, NB, P, WANT_W>), dim3((unsigned)grid), dim3(64), 0, s, a);
+  if constexpr (std::is_same<R, double>::value && OUT == OUT_V) {
+    if (a.list) {
+      const int64_t g = grid < kListGrid ? grid : kListGrid;
+      hipLaunchKernelGGL((k_zgeev_list<R, NB, P, OUT>), dim3((unsigned)g), dim3(64), 0, s, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_zgeev<R, NB, P, OUT>), dim3((unsigned)grid), dim3(64), 0, s, a);
 }
 
-template <typename R, bool WANT_W = false>
+template <typename R, int OUT = OUT_V>
 int launch_zgeev(const FusedArgs &a, hipStream_t s) {
   constexpr bool f32 = std::is_same<R, float>::value;
   const int nb = nb_for(a.G.n);
@@ -294,31 +387,31 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
   if (a.G.nmat > (int64_t)INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");
   switch (nb) {
     case 8:
-      if (p == 2) launch_one<R, 8, 2, WANT_W>(a, s);
-      else launch_one<R, 8, 1, WANT_W>(a, s);
+      if (p == 2) launch_one<R, 8, 2, OUT>(a, s);
+      else launch_one<R, 8, 1, OUT>(a, s);
       break;
     case 16:
       if constexpr (f32) {
         if (p == 4) {
-          launch_one<R, 16, 4, WANT_W>(a, s);
+          launch_one<R, 16, 4, OUT>(a, s);
           break;
         }
       }
-      if (p == 2) launch_one<R, 16, 2, WANT_W>(a, s);
-      else launch_one<R, 16, 1, WANT_W>(a, s);
+      if (p == 2) launch_one<R, 16, 2, OUT>(a, s);
+      else launch_one<R, 16, 1, OUT>(a, s);
       break;
     case 32:
       if constexpr (f32) {
         if (p == 2) {
-          launch_one<R, 32, 2, WANT_W>(a, s);
+          launch_one<R, 32, 2, OUT>(a, s);
           break;
         }
       }
-      launch_one<R, 32, 1, WANT_W>(a, s);
+      launch_one<R, 32, 1, OUT>(a, s);
       break;
     default:
-      if constexpr (!WANT_W) {
-        launch_one<R, 64, 1, false>(a, s);
+      if constexpr (OUT == OUT_V) {
+        launch_one<R, 64, 1, OUT_V>(a, s);
       } else {
         return fail(VEIG_ERR_UNSUPPORTED, "mixed precision handles orders up to 32");
       }

@@ -704,16 +704,13 @@ __device__ __forceinline__ void trevc_solve(const Group<R, NB, P>& g, int nmax, 
 }
 
 // --------------------------------------------------------------------------
-// Approximate inverse of the eigenvector matrix, W = X^{-1} Z^H, from the X
-// left in LDS by trevc_solve and the Schur-vector rows z[p][] of each lane.
-// The lane owning index k forms column k of Y = X^{-1} by back substitution
-// and stores its strict upper part transposed into the (unused) strict lower
-// triangle of the tile; then for each owned row r of Z it produces column r
-// of W, passed as out(p, k, W(k, r)).
+// Y = X^{-1} for the unit upper triangular X left in LDS by trevc_solve: the
+// lane owning index k forms column k of Y by back substitution and stores its
+// strict upper part transposed into the (unused) strict lower triangle of the
+// tile (Y(j,k), j < k, at tile position (k, j)).
 // --------------------------------------------------------------------------
-template <typename R, int NB, int P, typename Out>
-__device__ __forceinline__ void inverse_vectors(const Group<R, NB, P>& g, const cx<R> (&z)[P][NB],
-                                                int nmax, Out&& out) {
+template <typename R, int NB, int P>
+__device__ __forceinline__ void inverse_unit_upper(const Group<R, NB, P>& g, int nmax) {
   constexpr int LG = Group<R, NB, P>::LG;
   const int n = g.n;
 #pragma unroll
@@ -735,12 +732,23 @@ __device__ __forceinline__ void inverse_vectors(const Group<R, NB, P>& g, const 
         if (in && j < k) y[j] = mk(-acc.re, -acc.im);
       }
     }
-    // Y(j,k) (j < k) -> tile position (k, j)
 #pragma unroll
     for (int j = 0; j < NB; ++j)
       if (j < kcap && in && j < k) st<R>(g.at(k, j), y[j]);
   }
   wave_sync();
+}
+
+// --------------------------------------------------------------------------
+// Approximate inverse of the eigenvector matrix, W = X^{-1} Z^H = Y Z^H,
+// from Y (inverse_unit_upper) and the Schur-vector rows z[p][] of each lane:
+// for each owned row r of Z it produces column r of W, passed as
+// out(p, k, W(k, r)).
+// --------------------------------------------------------------------------
+template <typename R, int NB, int P, typename Out>
+__device__ __forceinline__ void inverse_vectors(const Group<R, NB, P>& g, const cx<R> (&z)[P][NB],
+                                                int nmax, Out&& out) {
+  inverse_unit_upper<R, NB, P>(g, nmax);
   // W(k, r) = conj(z[k]) + sum_{j>k} Y(k,j) conj(z[j])
 #pragma unroll
   for (int k = 0; k < NB; ++k) {

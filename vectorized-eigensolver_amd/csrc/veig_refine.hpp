@@ -95,13 +95,16 @@ __device__ __forceinline__ void cm_load(const RefTile<NT> &sh, CMat<NT> &x, int 
       }
 }
 
-// d += a * b  (a in A layout, b and d in B layout)
+// d += a * b  (a in A layout, b and d in B layout).  For one block the four
+// real products accumulate in four independent chains (two per part), summed
+// at the end, so that consecutive MFMAs do not wait on each other's result.
 template <int NT>
 __device__ __forceinline__ void cm_gemm(const CMat<NT> &a, const CMat<NT> &b, CMat<NT> &d) {
 #pragma unroll
   for (int I = 0; I < NT; ++I)
 #pragma unroll
-    for (int J = 0; J < NT; ++J)
+    for (int J = 0; J < NT; ++J) {
+      d4 r2 = d4{0, 0, 0, 0}, i2 = d4{0, 0, 0, 0};
 #pragma unroll
       for (int K = 0; K < NT; ++K)
 #pragma unroll
@@ -110,9 +113,19 @@ __device__ __forceinline__ void cm_gemm(const CMat<NT> &a, const CMat<NT> &b, CM
           const double br = b.re[K][J][s], bi = b.im[K][J][s];
           d.re[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, d.re[I][J], 0, 0, 0);
           d.im[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, d.im[I][J], 0, 0, 0);
-          d.re[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, d.re[I][J], 0, 0, 0);
-          d.im[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, d.im[I][J], 0, 0, 0);
+          if constexpr (NT == 1) {
+            r2 = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, r2, 0, 0, 0);
+            i2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, i2, 0, 0, 0);
+          } else {  // 2 x 2 blocks: registers are short, two chains
+            d.re[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, d.re[I][J], 0, 0, 0);
+            d.im[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, d.im[I][J], 0, 0, 0);
+          }
         }
+      if constexpr (NT == 1) {
+        d.re[I][J] += r2;
+        d.im[I][J] += i2;
+      }
+    }
 }
 
 __device__ __forceinline__ double wave_sum(double x) {
@@ -158,18 +171,172 @@ struct RefineArgs {
   Geom G;
   double tol;
   int maxit;                // 0 = measure the backward error only
+  // if set, only the matrices list[0 .. *list_count) (persistent grid)
+  const int32_t *list = nullptr;
+  const int32_t *list_count = nullptr;
+  // k_refine_zxy: the complex64 factors (Z in rW/iW; X above, X^{-1} below
+  // the diagonal in rX/iX) from which V and W are formed
+  const float *rX = nullptr, *iX = nullptr;
 };
 
+// Newton iterations on one matrix held by the whole wavefront: V (B layout)
+// and the eigenvalues of its columns (lc) and rows (lr) are updated in place;
+// returns the final backward error ||A V - V Lam||_F / (||A||_F ||V||_F).
+// load_a / load_w fill A and W in the A layout where they are needed;
+// to_a(V, X) transposes V into the A layout (through LDS); dl is LDS scratch
+// for the 16*NT eigenvalue corrections.
+template <int NT, typename LoadA, typename LoadW, typename ToA>
+__device__ __forceinline__ double newton_refine(CMat<NT> &V, cx<double> (&lc)[NT], cx<double> (&lr)[NT][4],
+                                                int n, double nrmA, double tol, int maxit, bool qr_ok,
+                                                double2 *dl, LoadA &&load_a, LoadW &&load_w, ToA &&to_a) {
+  const int l = threadIdx.x & 63, c = l & 15, g = l >> 4;
+  double be = INFINITY, be_prev = INFINITY;
+  for (int it = 0;; ++it) {
+    // ---- R = A V - V diag(lam), backward error ----
+    CMat<NT> Rm;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int J = 0; J < NT; ++J)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const double vr = V.re[I][J][s], vi = V.im[I][J][s];
+          Rm.re[I][J][s] = -(vr * lc[J].re - vi * lc[J].im);
+          Rm.im[I][J][s] = -(vr * lc[J].im + vi * lc[J].re);
+        }
+    {
+      CMat<NT> A;
+      load_a(A);
+      cm_gemm<NT>(A, V, Rm);
+    }
+    double rn2 = 0, vn2 = 0;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int J = 0; J < NT; ++J)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          rn2 += Rm.re[I][J][s] * Rm.re[I][J][s] + Rm.im[I][J][s] * Rm.im[I][J][s];
+          vn2 += V.re[I][J][s] * V.re[I][J][s] + V.im[I][J][s] * V.im[I][J][s];
+        }
+    rn2 = wave_sum(rn2);
+    vn2 = wave_sum(vn2);
+    be = sqrt(rn2) / (nrmA * sqrt(vn2));
+    if (n == 0) be = 0;
+    // wavefront-uniform stop: converged, out of iterations, or contracting
+    // too slowly to reach tol in the iterations left at the observed rate
+    // (the complex64 start is outside Newton's fast basin: the matrix goes
+    // to the FP64 solve instead)
+    if (!qr_ok || be <= tol || it >= maxit || !(be < be_prev)) break;
+    if (it > 0) {
+      const double rate = be / be_prev;
+      double reach = be;
+      for (int k = it; k < maxit; ++k) reach *= rate;
+      if (reach > tol) break;
+    }
+    be_prev = be;
+
+    // ---- E = W R ----
+    CMat<NT> Em;
+    cm_zero<NT>(Em);
+    {
+      CMat<NT> W;
+      load_w(W);
+      cm_gemm<NT>(W, Rm, Em);
+    }
+    // ---- dlam = diag(E); C = E / (lam_k - lam_i) off the diagonal ----
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (4 * s + g == c) dl[16 * I + c] = double2{Em.re[I][I][s], Em.im[I][I][s]};
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int J = 0; J < NT; ++J)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int row = 16 * I + 4 * s + g, col = 16 * J + c;
+          const cx<double> d = sub(lc[J], lr[I][s]);
+          const bool zero = row == col || row >= n || col >= n || (d.re == 0.0 && d.im == 0.0);
+          const cx<double> q = zero ? mk<double>(0, 0)
+                                    : cdiv(mk<double>(Em.re[I][J][s], Em.im[I][J][s]), d);
+          Em.re[I][J][s] = q.re;
+          Em.im[I][J][s] = q.im;
+        }
+    // ---- V <- V + V C ----
+    {
+      CMat<NT> X;
+      to_a(V, X);  // includes the wavefront syncs around the LDS round trip
+      cm_gemm<NT>(X, Em, V);
+    }
+#pragma unroll
+    for (int J = 0; J < NT; ++J) {
+      const double2 d = dl[16 * J + c];
+      lc[J] = add(lc[J], mk<double>(d.x, d.y));
+    }
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double2 d = dl[16 * I + 4 * s + g];
+        lr[I][s] = add(lr[I][s], mk<double>(d.x, d.y));
+      }
+    wave_sync();
+  }
+  return be;
+}
+
 template <int NT>
-__global__ __launch_bounds__(64) void k_refine_mfma(RefineArgs a) {
-  __shared__ RefTile<NT> sh;
-  const int64_t m = xcd_block();  // one matrix per wavefront
-  if (m >= a.G.nmat) return;
-  if (a.mask && a.mask[m] == 0) return;
-  const int l = threadIdx.x, c = l & 15, g = l >> 4;
+__device__ __forceinline__ double cm_norm2(const CMat<NT> &x) {
+  double a = 0;
+#pragma unroll
+  for (int I = 0; I < NT; ++I)
+#pragma unroll
+    for (int J = 0; J < NT; ++J)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) a += x.re[I][J][s] * x.re[I][J][s] + x.im[I][J][s] * x.im[I][J][s];
+  return wave_sum(a);
+}
+
+// V (B layout) and eigenvalues of one refined matrix to the batch layout
+template <int NT>
+__device__ __forceinline__ void store_refined(const Geom &G, int64_t base, int64_t vbase, double *rV, double *iV,
+                                              double *rE, double *iE, const CMat<NT> &V,
+                                              const cx<double> (&lc)[NT]) {
+  const int l = threadIdx.x & 63, c = l & 15, g = l >> 4;
+  const int n = G.n;
+#pragma unroll
+  for (int I = 0; I < NT; ++I)
+#pragma unroll
+    for (int J = 0; J < NT; ++J)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int row = 16 * I + 4 * s + g, col = 16 * J + c;
+        if (row < n && col < n) {
+          const int64_t idx = base + G.el(row, col);
+          rV[idx] = V.re[I][J][s];
+          iV[idx] = V.im[I][J][s];
+        }
+      }
+  if (g == 0) {
+#pragma unroll
+    for (int J = 0; J < NT; ++J) {
+      const int k = 16 * J + c;
+      if (k < n) {
+        rE[vbase + G.ve(k)] = lc[J].re;
+        iE[vbase + G.ve(k)] = lc[J].im;
+      }
+    }
+  }
+}
+
+// one matrix of the batch: V, E (and W) from global memory
+template <int NT>
+__device__ __forceinline__ void refine_one(const RefineArgs &a, RefTile<NT> &sh, int64_t m) {
+  const int l = threadIdx.x & 63, c = l & 15, g = l >> 4;
   const int n = a.G.n;
   const int64_t base = a.G.mat_base(m), vbase = a.G.vec_base(m);
-
   // A and W stay in registers for one block; for larger orders they are
   // re-read (from L2) where they are used, to stay within 256 VGPRs.
   constexpr bool kHold = NT == 1;
@@ -177,14 +344,7 @@ __global__ __launch_bounds__(64) void k_refine_mfma(RefineArgs a) {
   cm_load_global<NT, LAY_A>(A, a.G, base, a.rA, a.iA, c, g);
   cm_load_global<NT, LAY_B>(V, a.G, base, a.rV, a.iV, c, g);
   if (kHold && a.maxit > 0) cm_load_global<NT, LAY_A>(W, a.G, base, a.rW, a.iW, c, g);
-  double an2 = 0;
-#pragma unroll
-  for (int I = 0; I < NT; ++I)
-#pragma unroll
-    for (int J = 0; J < NT; ++J)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) an2 += A.re[I][J][s] * A.re[I][J][s] + A.im[I][J][s] * A.im[I][J][s];
-  const double nrmA = sqrt(wave_sum(an2));
+  const double nrmA = sqrt(cm_norm2<NT>(A));
   // eigenvalues of the B-layout columns (lc) and rows (lr)
   cx<double> lc[NT], lr[NT][4];
 #pragma unroll
@@ -200,122 +360,362 @@ __global__ __launch_bounds__(64) void k_refine_mfma(RefineArgs a) {
       lr[I][s] = k < n ? mk<double>(a.rE[vbase + a.G.ve(k)], a.iE[vbase + a.G.ve(k)]) : mk<double>(0, 0);
     }
   const bool qr_ok = !(a.info32 && a.info32[m] != 0);
-  double be = INFINITY;
-  for (int it = 0;; ++it) {
-    // ---- R = A V - V diag(lam), backward error ----
-    CMat<NT> Rm;
-#pragma unroll
-    for (int I = 0; I < NT; ++I)
-#pragma unroll
-      for (int J = 0; J < NT; ++J)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const double vr = V.re[I][J][s], vi = V.im[I][J][s];
-          Rm.re[I][J][s] = -(vr * lc[J].re - vi * lc[J].im);
-          Rm.im[I][J][s] = -(vr * lc[J].im + vi * lc[J].re);
+  const double be = newton_refine<NT>(
+      V, lc, lr, n, nrmA, a.tol, a.maxit, qr_ok, sh.dl,
+      [&](CMat<NT> &x) {
+        if constexpr (kHold) {
+          x = A;
+        } else {
+          int64_t b = base;
+          asm volatile("" : "+v"(b));  // opaque: keep the reload inside the loop
+          cm_load_global<NT, LAY_A>(x, a.G, b, a.rA, a.iA, c, g);
         }
-    if constexpr (!kHold) {
-      int64_t b = base;
-      asm volatile("" : "+v"(b));  // opaque: keep the reload inside the loop
-      cm_load_global<NT, LAY_A>(A, a.G, b, a.rA, a.iA, c, g);
-    }
-    cm_gemm<NT>(A, V, Rm);
-    double rn2 = 0, vn2 = 0;
-#pragma unroll
-    for (int I = 0; I < NT; ++I)
-#pragma unroll
-      for (int J = 0; J < NT; ++J)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          rn2 += Rm.re[I][J][s] * Rm.re[I][J][s] + Rm.im[I][J][s] * Rm.im[I][J][s];
-          vn2 += V.re[I][J][s] * V.re[I][J][s] + V.im[I][J][s] * V.im[I][J][s];
+      },
+      [&](CMat<NT> &x) {
+        if constexpr (kHold) {
+          x = W;
+        } else {
+          int64_t b = base;
+          asm volatile("" : "+v"(b));
+          cm_load_global<NT, LAY_A>(x, a.G, b, a.rW, a.iW, c, g);
         }
-    rn2 = wave_sum(rn2);
-    vn2 = wave_sum(vn2);
-    be = sqrt(rn2) / (nrmA * sqrt(vn2));
-    if (n == 0) be = 0;
-    if (!qr_ok || be <= a.tol || it >= a.maxit) break;  // wavefront-uniform
-
-    // ---- E = W R ----
-    CMat<NT> Em;
-    cm_zero<NT>(Em);
-    if constexpr (!kHold) {
-      int64_t b = base;
-      asm volatile("" : "+v"(b));
-      cm_load_global<NT, LAY_A>(W, a.G, b, a.rW, a.iW, c, g);
-    }
-    cm_gemm<NT>(W, Rm, Em);
-    // ---- dlam = diag(E); C = E / (lam_k - lam_i) off the diagonal ----
-#pragma unroll
-    for (int I = 0; I < NT; ++I)
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-        if (4 * s + g == c) sh.dl[16 * I + c] = double2{Em.re[I][I][s], Em.im[I][I][s]};
-#pragma unroll
-    for (int I = 0; I < NT; ++I)
-#pragma unroll
-      for (int J = 0; J < NT; ++J)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int row = 16 * I + 4 * s + g, col = 16 * J + c;
-          const cx<double> d = sub(lc[J], lr[I][s]);
-          const bool zero = row == col || row >= n || col >= n || (d.re == 0.0 && d.im == 0.0);
-          const cx<double> q = zero ? mk<double>(0, 0)
-                                    : cdiv(mk<double>(Em.re[I][J][s], Em.im[I][J][s]), d);
-          Em.re[I][J][s] = q.re;
-          Em.im[I][J][s] = q.im;
-        }
-    // ---- V <- V + V C (V transposed to the A layout through LDS) ----
-    CMat<NT> X;
-    cm_store<NT, LAY_B>(sh, V, c, g);
-    wave_sync();
-    cm_load<NT, LAY_A>(sh, X, c, g);
-    cm_gemm<NT>(X, Em, V);
-#pragma unroll
-    for (int J = 0; J < NT; ++J) {
-      const double2 d = sh.dl[16 * J + c];
-      lc[J] = add(lc[J], mk<double>(d.x, d.y));
-    }
-#pragma unroll
-    for (int I = 0; I < NT; ++I)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const double2 d = sh.dl[16 * I + 4 * s + g];
-        lr[I][s] = add(lr[I][s], mk<double>(d.x, d.y));
-      }
-    wave_sync();
-  }
+      },
+      [&](const CMat<NT> &v, CMat<NT> &x) {
+        cm_store<NT, LAY_B>(sh, v, c, g);
+        wave_sync();
+        cm_load<NT, LAY_A>(sh, x, c, g);
+      });
   const bool ok = (be <= a.tol) && qr_ok;
   if (a.maxit > 0) {
-#pragma unroll
-    for (int I = 0; I < NT; ++I)
-#pragma unroll
-      for (int J = 0; J < NT; ++J)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int row = 16 * I + 4 * s + g, col = 16 * J + c;
-          if (row < n && col < n) {
-            const int64_t idx = base + a.G.el(row, col);
-            a.rV[idx] = V.re[I][J][s];
-            a.iV[idx] = V.im[I][J][s];
-          }
-        }
-    if (g == 0) {
-#pragma unroll
-      for (int J = 0; J < NT; ++J) {
-        const int k = 16 * J + c;
-        if (k < n) {
-          a.rE[vbase + a.G.ve(k)] = lc[J].re;
-          a.iE[vbase + a.G.ve(k)] = lc[J].im;
-        }
-      }
-    }
+    store_refined<NT>(a.G, base, vbase, a.rV, a.iV, a.rE, a.iE, V, lc);
     if (l == 0) {
       a.flags[m] = ok ? 0 : 1;
       if (a.info && ok) a.info[m] = 0;
     }
   }
   if (l == 0 && a.berr) a.berr[m] = be;
+}
+
+// Workgroups of kRefWaves wavefronts refine kRefWaves consecutive matrices:
+// in the tiled layout the same element of consecutive matrices shares a
+// cache line, so the wavefronts of one workgroup (one CU) reuse each line
+// from L1 instead of each issuing its own L2 request per element.
+constexpr int kRefWaves = 4;
+
+template <int NT>
+__global__ __launch_bounds__(64 * kRefWaves) void k_refine_mfma(RefineArgs a) {
+  __shared__ RefTile<NT> sh[kRefWaves];
+  const int w = threadIdx.x / 64;
+  if (a.list) {
+    const int64_t cnt = *a.list_count;
+    for (int64_t i = (int64_t)blockIdx.x * kRefWaves + w; i < cnt; i += (int64_t)gridDim.x * kRefWaves) {
+      refine_one<NT>(a, sh[w], a.list[i]);
+      wave_sync();
+    }
+    return;
+  }
+  const int64_t m = xcd_block() * kRefWaves + w;
+  if (m >= a.G.nmat) return;
+  if (a.mask && a.mask[m] == 0) return;
+  refine_one<NT>(a, sh[w], m);
+}
+
+// ---------------------------------------------------------------------------
+// Orders up to 16, whole batch: workgroups of kStageWaves wavefronts refine
+// kStageWaves consecutive matrices and move A, V, W in and V out through LDS
+// with a coalesced pattern (consecutive threads take the same element of
+// consecutive matrices, which the tiled layout keeps in one cache segment),
+// instead of one scattered 8-byte access per lane and element.
+// ---------------------------------------------------------------------------
+constexpr int kStageWaves = 4;
+constexpr int kStageLd = 16 * 16 + 1;  // per-matrix slice (+1: spreads the slices over the LDS banks)
+
+struct StagedShared {
+  double2 st[kStageWaves * kStageLd];
+  double2 dl[kStageWaves][16];
+};
+
+// element e (column-major) of matrix j of the workgroup -> slice j
+template <typename T>
+__device__ __forceinline__ void stage_in(StagedShared &sh, const Geom &G, int64_t mbase, bool have, const T *re,
+                                         const T *im) {
+  const int n = G.n, nn = n * n;
+  const int j = threadIdx.x % kStageWaves;
+  for (int e = threadIdx.x / kStageWaves; e < nn; e += blockDim.x / kStageWaves) {
+    const int row = e % n, col = e / n;
+    double2 v = double2{0, 0};
+    if (have) {
+      const int64_t idx = mbase + G.el(row, col);
+      v = double2{(double)re[idx], (double)im[idx]};
+    }
+    sh.st[j * kStageLd + col * 16 + row] = v;
+  }
+}
+
+__device__ __forceinline__ void stage_out(const StagedShared &sh, const Geom &G, int64_t mbase, bool have, double *re,
+                                          double *im) {
+  const int n = G.n, nn = n * n;
+  const int j = threadIdx.x % kStageWaves;
+  for (int e = threadIdx.x / kStageWaves; e < nn; e += blockDim.x / kStageWaves) {
+    const int row = e % n, col = e / n;
+    if (have) {
+      const int64_t idx = mbase + G.el(row, col);
+      const double2 v = sh.st[j * kStageLd + col * 16 + row];
+      re[idx] = v.x;
+      im[idx] = v.y;
+    }
+  }
+}
+
+// a wavefront's 16 x 16 operand from / to its slice (zero outside n on load)
+template <int LAY>
+__device__ __forceinline__ void slice_load(const double2 *sl, CMat<1> &x, int n, int c, int g) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    int row, col;
+    lay_rc<LAY>(0, 0, s, c, g, row, col);
+    double2 v = double2{0, 0};
+    if (row < n && col < n) v = sl[col * 16 + row];
+    x.re[0][0][s] = v.x;
+    x.im[0][0][s] = v.y;
+  }
+}
+
+template <int LAY>
+__device__ __forceinline__ void slice_store(double2 *sl, const CMat<1> &x, int c, int g) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    int row, col;
+    lay_rc<LAY>(0, 0, s, c, g, row, col);
+    sl[col * 16 + row] = double2{x.re[0][0][s], x.im[0][0][s]};
+  }
+}
+
+__global__ __launch_bounds__(64 * kStageWaves) void k_refine_staged(RefineArgs a) {
+  __shared__ StagedShared sh;
+  const int w = threadIdx.x / 64, l = threadIdx.x & 63, c = l & 15, g = l >> 4;
+  const int n = a.G.n;
+  const int64_t m0 = xcd_block() * kStageWaves;
+  // this thread's staging matrix (threadIdx % kStageWaves) and this wavefront's matrix (w)
+  const int js = threadIdx.x % kStageWaves;
+  const bool have_s = m0 + js < a.G.nmat;
+  const int64_t base_s = have_s ? a.G.mat_base(m0 + js) : 0;
+  const int64_t m = m0 + w;
+  const bool have = m < a.G.nmat;
+  double2 *sl = sh.st + w * kStageLd;
+
+  CMat<1> A, V, W;
+  stage_in(sh, a.G, base_s, have_s, a.rA, a.iA);
+  __syncthreads();
+  slice_load<LAY_A>(sl, A, n, c, g);
+  __syncthreads();
+  stage_in(sh, a.G, base_s, have_s, a.rV, a.iV);
+  __syncthreads();
+  slice_load<LAY_B>(sl, V, n, c, g);
+  __syncthreads();
+  if (a.maxit > 0) {
+    stage_in(sh, a.G, base_s, have_s, a.rW, a.iW);
+    __syncthreads();
+    slice_load<LAY_A>(sl, W, n, c, g);
+    __syncthreads();
+  }
+  double be = 0;
+  bool ok = false;
+  cx<double> lc[1], lr[1][4];
+  if (have) {
+    const int64_t vbase = a.G.vec_base(m);
+    const double nrmA = sqrt(cm_norm2<1>(A));
+    lc[0] = c < n ? mk<double>(a.rE[vbase + a.G.ve(c)], a.iE[vbase + a.G.ve(c)]) : mk<double>(0, 0);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * s + g;
+      lr[0][s] = k < n ? mk<double>(a.rE[vbase + a.G.ve(k)], a.iE[vbase + a.G.ve(k)]) : mk<double>(0, 0);
+    }
+    const bool qr_ok = !(a.info32 && a.info32[m] != 0);
+    be = newton_refine<1>(
+        V, lc, lr, n, nrmA, a.tol, a.maxit, qr_ok, sh.dl[w], [&](CMat<1> &x) { x = A; },
+        [&](CMat<1> &x) { x = W; },
+        [&](const CMat<1> &v, CMat<1> &x) {
+          slice_store<LAY_B>(sl, v, c, g);
+          wave_sync();
+          slice_load<LAY_A>(sl, x, 16, c, g);
+        });
+    ok = (be <= a.tol) && qr_ok;
+    wave_sync();
+    slice_store<LAY_B>(sl, V, c, g);
+  }
+  __syncthreads();
+  if (a.maxit > 0) {
+    stage_out(sh, a.G, base_s, have_s, a.rV, a.iV);
+    if (have) {
+      const int64_t vbase = a.G.vec_base(m);
+      if (g == 0 && c < n) {
+        a.rE[vbase + a.G.ve(c)] = lc[0].re;
+        a.iE[vbase + a.G.ve(c)] = lc[0].im;
+      }
+      if (l == 0) {
+        a.flags[m] = ok ? 0 : 1;
+        if (a.info && ok) a.info[m] = 0;
+      }
+    }
+  }
+  if (have && l == 0 && a.berr) a.berr[m] = be;
+}
+
+// ---------------------------------------------------------------------------
+// Orders up to 16 from the complex64 factors: V = Z X and W = X^{-1} Z^H are
+// formed on the matrix cores (FP64 products of complex64 data), then refined.
+// A workgroup of kStageWaves wavefronts takes kStageWaves consecutive
+// matrices; Z, X | X^{-1}, A and the eigenvalues come in through LDS in one
+// coalesced phase (one memory latency per workgroup), V goes out the same way.
+// ---------------------------------------------------------------------------
+struct ZxyShared {
+  double2 z[kStageWaves * kStageLd];   // Z; then per-wavefront transpose scratch
+  double2 xy[kStageWaves * kStageLd];  // X | X^{-1}; then the refined V
+  double2 a[kStageWaves * kStageLd];   // A
+  double2 e[kStageWaves][16];          // eigenvalues
+  double2 dl[kStageWaves][16];
+};
+
+__global__ __launch_bounds__(64 * kStageWaves) void k_refine_zxy(RefineArgs a) {
+  __shared__ ZxyShared sh;
+  const int w = threadIdx.x / 64, l = threadIdx.x & 63, c = l & 15, g = l >> 4;
+  const int n = a.G.n, nn = n * n;
+  const int64_t m0 = xcd_block() * kStageWaves;
+  // ---- one staging phase: consecutive threads take consecutive matrices ----
+  {
+    const int js = threadIdx.x % kStageWaves;
+    const bool have_s = m0 + js < a.G.nmat;
+    const int64_t base_s = have_s ? a.G.mat_base(m0 + js) : 0;
+    constexpr int kStep = 64;  // elements per pass: blockDim / kStageWaves
+    constexpr int kPass = 256 / kStep;
+    double2 vz[kPass], vx[kPass], va[kPass];
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {
+      const int e = threadIdx.x / kStageWaves + k * kStep;
+      vz[k] = vx[k] = va[k] = double2{0, 0};
+      if (have_s && e < nn) {
+        const int64_t idx = base_s + a.G.el(e % n, e / n);
+        vz[k] = double2{(double)a.rW[idx], (double)a.iW[idx]};
+        vx[k] = double2{(double)a.rX[idx], (double)a.iX[idx]};
+        va[k] = double2{a.rA[idx], a.iA[idx]};
+      }
+    }
+    if (threadIdx.x < kStageWaves * 16) {
+      const int r = threadIdx.x / kStageWaves;
+      double2 ev = double2{0, 0};
+      if (have_s && r < n) {
+        const int64_t vi = a.G.vec_base(m0 + js) + a.G.ve(r);
+        ev = double2{a.rE[vi], a.iE[vi]};
+      }
+      sh.e[js][r] = ev;
+    }
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {
+      const int e = threadIdx.x / kStageWaves + k * kStep;
+      if (e < nn) {
+        const int o = js * kStageLd + (e / n) * 16 + e % n;
+        sh.z[o] = vz[k];
+        sh.xy[o] = vx[k];
+        sh.a[o] = va[k];
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t m = m0 + w;
+  const bool have = m < a.G.nmat;
+  double2 *zs = sh.z + w * kStageLd, *xs = sh.xy + w * kStageLd, *as = sh.a + w * kStageLd;
+  double be = 0;
+  bool ok = false;
+  cx<double> lc[1];
+  if (have) {
+    // ---- V = Z X, W = Y Z^H (Y = X^{-1}) ----
+    CMat<1> V, W;
+    {
+      CMat<1> Za, Xb, Ya, Zh;
+      slice_load<LAY_A>(zs, Za, n, c, g);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        {  // X in the B layout: unit upper triangular
+          const int row = 4 * s + g, col = c;
+          double2 v = double2{0, 0};
+          if (row < n && col < n) v = row < col ? xs[col * 16 + row] : (row == col ? double2{1, 0} : v);
+          Xb.re[0][0][s] = v.x;
+          Xb.im[0][0][s] = v.y;
+        }
+        {  // Y in the A layout: Y(row, col), row < col, is stored at (col, row)
+          const int row = c, col = 4 * s + g;
+          double2 v = double2{0, 0};
+          if (row < n && col < n) v = row < col ? xs[row * 16 + col] : (row == col ? double2{1, 0} : v);
+          Ya.re[0][0][s] = v.x;
+          Ya.im[0][0][s] = v.y;
+        }
+        Zh.re[0][0][s] = Za.re[0][0][s];  // Z^H in the B layout = conj(Z in the A layout)
+        Zh.im[0][0][s] = -Za.im[0][0][s];
+      }
+      cm_zero<1>(V);
+      cm_gemm<1>(Za, Xb, V);
+      cm_zero<1>(W);
+      cm_gemm<1>(Ya, Zh, W);
+    }
+    wave_sync();  // Z read: its slice becomes this wavefront's transpose scratch
+    CMat<1> Wa;
+    slice_store<LAY_B>(zs, W, c, g);
+    wave_sync();
+    slice_load<LAY_A>(zs, Wa, 16, c, g);
+    // ---- Newton refinement ----
+    CMat<1> A;
+    slice_load<LAY_A>(as, A, n, c, g);
+    const double nrmA = sqrt(cm_norm2<1>(A));
+    cx<double> lr[1][4];
+    {
+      const double2 v = sh.e[w][c];
+      lc[0] = mk<double>(v.x, v.y);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const double2 v = sh.e[w][4 * s + g];
+      lr[0][s] = mk<double>(v.x, v.y);
+    }
+    const bool qr_ok = !(a.info32 && a.info32[m] != 0);
+    wave_sync();
+    be = newton_refine<1>(
+        V, lc, lr, n, nrmA, a.tol, a.maxit, qr_ok, sh.dl[w], [&](CMat<1> &x) { x = A; },
+        [&](CMat<1> &x) { x = Wa; },
+        [&](const CMat<1> &v, CMat<1> &x) {
+          slice_store<LAY_B>(zs, v, c, g);
+          wave_sync();
+          slice_load<LAY_A>(zs, x, 16, c, g);
+        });
+    ok = (be <= a.tol) && qr_ok;
+    slice_store<LAY_B>(xs, V, c, g);  // X | Y were read before the refinement
+  }
+  __syncthreads();
+  {
+    const int js = threadIdx.x % kStageWaves;
+    const bool have_s = m0 + js < a.G.nmat;
+    if (have_s) {
+      const int64_t base_s = a.G.mat_base(m0 + js);
+      for (int e = threadIdx.x / kStageWaves; e < nn; e += 64) {
+        const double2 v = sh.xy[js * kStageLd + (e / n) * 16 + e % n];
+        const int64_t idx = base_s + a.G.el(e % n, e / n);
+        a.rV[idx] = v.x;
+        a.iV[idx] = v.y;
+      }
+    }
+  }
+  if (have) {
+    const int64_t vbase = a.G.vec_base(m);
+    if (g == 0 && c < n) {
+      a.rE[vbase + a.G.ve(c)] = lc[0].re;
+      a.iE[vbase + a.G.ve(c)] = lc[0].im;
+    }
+    if (l == 0) {
+      a.flags[m] = ok ? 0 : 1;
+      if (a.info && ok) a.info[m] = 0;
+      if (a.berr) a.berr[m] = be;
+    }
+  }
 }
 
 }  // namespace veig
