@@ -63,12 +63,13 @@ def test_mixed_hermitian(gpu, n):
     check(A, E, V, info, fb, be, 1e-12)
 
 
-@pytest.mark.parametrize("cond", [1e1, 1e3, 1e6])
-def test_mixed_conditioned(gpu, cond):
+@pytest.mark.parametrize("n,cond", [(24, 1e1), (24, 1e3), (24, 1e6), (16, 1e3), (16, 1e6), (8, 1e6)])
+def test_mixed_conditioned(gpu, n, cond):
     # configuration of the mixed-precision workload: A = V diag(lam) V^-1
     # with cond(V) up to 1e6; matrices the refinement cannot bring to the
-    # tolerance fall back to FP64, so the bar holds for all of them
-    A = constructed(128, 24, 5200 + int(np.log10(cond)), cond=cond)
+    # tolerance fall back to FP64, so the bar holds for all of them (n <= 16
+    # refines from the complex64 Schur factors, n > 16 from V and W)
+    A = constructed(128, n, 5200 + 10 * n + int(np.log10(cond)), cond=cond)
     E, V, info, fb, be = run_mixed(A)
     check(A, E, V, info, fb, be, 1e-12, cond=cond)
 

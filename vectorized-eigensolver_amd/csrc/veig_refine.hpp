@@ -95,43 +95,75 @@ __device__ __forceinline__ void cm_load(const RefTile<NT> &sh, CMat<NT> &x, int 
       }
 }
 
-// d += a * b  (a in A layout, b and d in B layout).  For one block the four
-// real products accumulate in four independent chains (two per part), summed
-// at the end, so that consecutive MFMAs do not wait on each other's result.
+// d += a * b  (a in A layout, b and d in B layout).  One block: the
+// three-product form of the complex product, P1 = ar br, P2 = ai bi,
+// P3 = (ar + ai)(br + bi), re += P1 - P2, im += P3 - P1 - P2 (12 MFMAs
+// instead of 16; the error bound grows by at most a factor of about two),
+// the three products in independent chains.
 template <int NT>
 __device__ __forceinline__ void cm_gemm(const CMat<NT> &a, const CMat<NT> &b, CMat<NT> &d) {
+  if constexpr (NT > 1) {
+    // 2 x 2 blocks: registers are short; four products in two chains
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+      for (int J = 0; J < NT; ++J)
+#pragma unroll
+        for (int K = 0; K < NT; ++K)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const double ar = a.re[I][K][s], ai = a.im[I][K][s];
+            const double br = b.re[K][J][s], bi = b.im[K][J][s];
+            d.re[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, d.re[I][J], 0, 0, 0);
+            d.im[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, d.im[I][J], 0, 0, 0);
+            d.re[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, d.re[I][J], 0, 0, 0);
+            d.im[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, d.im[I][J], 0, 0, 0);
+          }
+    return;
+  }
 #pragma unroll
   for (int I = 0; I < NT; ++I)
 #pragma unroll
     for (int J = 0; J < NT; ++J) {
-      d4 r2 = d4{0, 0, 0, 0}, i2 = d4{0, 0, 0, 0};
+      d4 p1 = d4{0, 0, 0, 0}, p2 = d4{0, 0, 0, 0}, p3 = d4{0, 0, 0, 0};
 #pragma unroll
       for (int K = 0; K < NT; ++K)
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const double ar = a.re[I][K][s], ai = a.im[I][K][s];
           const double br = b.re[K][J][s], bi = b.im[K][J][s];
-          d.re[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, d.re[I][J], 0, 0, 0);
-          d.im[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, d.im[I][J], 0, 0, 0);
-          if constexpr (NT == 1) {
-            r2 = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, r2, 0, 0, 0);
-            i2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, i2, 0, 0, 0);
-          } else {  // 2 x 2 blocks: registers are short, two chains
-            d.re[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, d.re[I][J], 0, 0, 0);
-            d.im[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, d.im[I][J], 0, 0, 0);
-          }
+          p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, p1, 0, 0, 0);
+          p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, p2, 0, 0, 0);
+          p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, p3, 0, 0, 0);
         }
-      if constexpr (NT == 1) {
-        d.re[I][J] += r2;
-        d.im[I][J] += i2;
-      }
+      d.re[I][J] += p1 - p2;
+      d.im[I][J] += p3 - p1 - p2;
     }
 }
 
+// DPP move of a double (both halves with the same control)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)__double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)__double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_f64(double x, int lane) {
+  const int lo = __builtin_amdgcn_readlane((int)__double2loint(x), lane);
+  const int hi = __builtin_amdgcn_readlane((int)__double2hiint(x), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront, returned in every lane: DPP sums
+// within each row of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+// row_mirror), then the four row sums are read as scalars.
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+  x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return (lane_f64(x, 0) + lane_f64(x, 16)) + (lane_f64(x, 32) + lane_f64(x, 48));
 }
 
 // load a matrix of the tiled batch layout (re/im planes of type T) into
@@ -258,9 +290,11 @@ __device__ __forceinline__ double newton_refine(CMat<NT> &V, cx<double> (&lc)[NT
         for (int s = 0; s < 4; ++s) {
           const int row = 16 * I + 4 * s + g, col = 16 * J + c;
           const cx<double> d = sub(lc[J], lr[I][s]);
-          const bool zero = row == col || row >= n || col >= n || (d.re == 0.0 && d.im == 0.0);
+          const double d2 = abs2(d);
+          const bool zero = row == col || row >= n || col >= n || !(d2 > 0.0);
+          // E / d = E conj(d) / |d|^2 (|d|^2 underflowing: no correction)
           const cx<double> q = zero ? mk<double>(0, 0)
-                                    : cdiv(mk<double>(Em.re[I][J][s], Em.im[I][J][s]), d);
+                                    : scale(mulc(mk<double>(Em.re[I][J][s], Em.im[I][J][s]), d), frcp(d2));
           Em.re[I][J][s] = q.re;
           Em.im[I][J][s] = q.im;
         }
@@ -577,7 +611,7 @@ struct ZxyShared {
   double2 dl[kStageWaves][16];
 };
 
-__global__ __launch_bounds__(64 * kStageWaves) void k_refine_zxy(RefineArgs a) {
+__global__ __launch_bounds__(64 * kStageWaves) __attribute__((amdgpu_waves_per_eu(3))) void k_refine_zxy(RefineArgs a) {
   __shared__ ZxyShared sh;
   const int w = threadIdx.x / 64, l = threadIdx.x & 63, c = l & 15, g = l >> 4;
   const int n = a.G.n, nn = n * n;
