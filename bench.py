@@ -172,10 +172,10 @@ def bench_uniform(a, cfg, veig, W, torch, dev, stream):
     nbytes = N * alg_bytes(n)
     if a.config == "c4" and mixed is not None:
         value, ms, dtype = mixed["value"], mixed["ms_per_step"], "c64+f64"
-        rf = roofline(flops, nbytes, mixed["fp32_kernel_ms"], f"k_zgeev<float,{nb(n)},W>", f"zgeev_fp32_n{n}")
+        rf = roofline(flops, nbytes, mixed["fp32_kernel_ms"], f"k_zgeev<float, {nb(n)}, 1, 0>", f"zgeev_fp32_n{n}")
     else:
         value, ms, dtype = v64, ms64, "f64"
-        rf = roofline(flops, nbytes, ms64, f"k_zgeev<double,{nb(n)}>", f"zgeev_fp64_n{n}")
+        rf = roofline(flops, nbytes, ms64, f"k_zgeev<double, {nb(n)}, 1, 0>", f"zgeev_fp64_n{n}")
     rf["flops_per_matrix"] = nominal_flops(n)
 
     if not a.no_cpu:
@@ -195,7 +195,7 @@ def bench_uniform(a, cfg, veig, W, torch, dev, stream):
 
 
 def nb(n):
-    return 8 if n <= 8 else (16 if n <= 16 else 32)
+    return 8 if n <= 8 else (16 if n <= 16 else (32 if n <= 32 else 64))
 
 
 def sample_check(rA, iA, rE, iE, rV, iV, n, T, N, info, torch):
