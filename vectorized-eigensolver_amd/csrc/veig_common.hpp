@@ -82,8 +82,14 @@ struct Tiles {
   using L2 = typename lds2<R>::type;
   static constexpr int MPW = Group<R, NB, P>::MPW;
   static constexpr int LD = NB + 1;
-  L2 H[MPW * NB * LD];
-  L2 v[MPW * NB];
+  // one element of padding between the groups' tiles (and vectors) starts
+  // each group on a different LDS bank: the groups' broadcast reads of their
+  // own matrices (reflector inputs, shifts, T(i,j) in the solves) would
+  // otherwise all hit the same bank with different addresses
+  static constexpr int kTile = NB * LD + 1;
+  static constexpr int kVec = NB + 1;
+  L2 H[MPW * kTile];
+  L2 v[MPW * kVec];
 };
 
 template <typename R, int NB, int P = 1>
@@ -94,8 +100,8 @@ __device__ __forceinline__ Group<R, NB, P> make_group(Tiles<R, NB, P> &sm, int n
   g.mi = lane / G::LG;
   g.r = lane % G::LG;
   g.n = n_of_matrix;
-  g.H = sm.H + g.mi * NB * G::LD;
-  g.vsh = sm.v + g.mi * NB;
+  g.H = sm.H + g.mi * Tiles<R, NB, P>::kTile;
+  g.vsh = sm.v + g.mi * Tiles<R, NB, P>::kVec;
   // zero the whole tile: entries outside the matrix (and below the
   // subdiagonal when only a Hessenberg part is loaded) read as exact zeros
 #pragma unroll
