@@ -321,6 +321,10 @@ def bench_varsize(a, cfg, veig, W, torch, dev, stream):
     vb = veig.VarBatch(dims.cpu(), device=dev)
     W.fill_c3(vb, dims, kind, cfg["seed"], dev)
     torch.cuda.synchronize()
+    progress("autotuning kernel variant per order and grids (untimed)")
+    variants, grids = vb.autotune()
+    W.fill_c3(vb, dims, kind, cfg["seed"], dev)  # the tuner overwrote the outputs only; refill anyway
+    torch.cuda.synchronize()
     g = vb.graph()
     progress("timing the captured graph")
     ms, wall = timed(lambda: g.launch(stream), a.steps, a.warmup, stream)
@@ -328,7 +332,7 @@ def bench_varsize(a, cfg, veig, W, torch, dev, stream):
     u, c = torch.unique(dims.cpu(), return_counts=True)
     flops = float(sum(nominal_flops(int(n)) * int(k) for n, k in zip(u, c)))
     nbytes = int(sum(alg_bytes(int(n)) * int(k) for n, k in zip(u, c)))
-    rf = roofline(flops, nbytes, ms, "graph: bin kernels + k_zgeev_var<8|16|32>", "varsize_c3")
+    rf = roofline(flops, nbytes, ms, "graph: bin kernels + k_zgeev_var<NB,P> per variant", "varsize_c3")
     rf["note"] = "whole graph timed (binning + three persistent bin kernels)"
     info = vb.info
     # backward error on a sample of every order
@@ -344,7 +348,10 @@ def bench_varsize(a, cfg, veig, W, torch, dev, stream):
         "unit": "matrices/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": workload, "name": "c3", "batch": N},
+        "config": {"workload": workload, "name": "c3", "batch": N,
+                   "autotuned": {"variant_of_n": variants[1:], "grid_per_variant": grids,
+                                 "variants": "0: 8 lanes/matrix (width 8), 1: 4 (width 8, 2 per lane), "
+                                             "2: 16 (width 16), 3: 8 (width 16, 2 per lane), 4: 32"}},
         "roofline": rf, "cpu_baseline": cpu, "mixed": None,
         "fp64": {"value": round(value, 1), "ms_per_step": round(ms, 4),
                  "unconverged_matrices": int((info > 0).sum()), **chk},

@@ -101,9 +101,15 @@ int veig_zgeev_mixed(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const d
  * Matrix m (order dims[m], 0 <= dims[m] <= 32) is column-major with leading
  * dimension dims[m] at offset a_off[m] of the rA/iA planes; its eigenvectors
  * go to the same offsets of rV/iV and its eigenvalues to e_off[m] of rE/iE.
- * The matrices are sorted into lane-group widths NB = 8/16/32 on the device
- * (nb_of_n[n], may be NULL for the smallest width >= n) and solved by one
- * persistent kernel per width (grid_per_bin[3] workgroups, NULL = default).
+ * The matrices are sorted on the device by the kernel variant chosen for
+ * their order, variant_of_n[n] (33 int8 in device memory, may be NULL for the
+ * defaults): 0 = lane groups of 8 (one lane per index), 1 = groups of 8 with
+ * 2 indices per lane (4 lanes per matrix), 2 = groups of 16, 3 = groups of 16
+ * with 2 indices per lane (8 lanes per matrix), 4 = groups of 32; a variant
+ * too narrow for n is replaced by the default (0 for n <= 8, 2 for n <= 16,
+ * 4 above).  One persistent kernel per variant (grid_per_bin[5] workgroups,
+ * a host array read when the launches are enqueued or captured; NULL or 0 =
+ * default).
  * info[m] (may be NULL): as veig_zgeev, -1 for an order outside [0, 32].
  * Enqueues only kernels (no allocation, copy or memset), so it can be
  * captured into a graph; veig_varsize_graph_create does that.
@@ -114,15 +120,28 @@ size_t veig_varsize_workspace_size(int64_t nmat);
 int veig_zgeev_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off,
                        const int64_t *e_off, const double *rA, const double *iA, double *rE,
                        double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
-                       const int8_t *nb_of_n, const int32_t *grid_per_bin, void *workspace,
+                       const int8_t *variant_of_n, const int32_t *grid_per_bin, void *workspace,
                        size_t workspace_bytes, void *stream);
 /* Capture the same launch sequence into an instantiated hipGraph (arguments as
  * above; pointers are baked in, array contents are read at each launch). */
 int veig_varsize_graph_create(int64_t nmat, const int32_t *dims, const int64_t *a_off,
                               const int64_t *e_off, const double *rA, const double *iA,
                               double *rE, double *iE, double *rV, double *iV, int32_t *info,
-                              int32_t itmax, const int8_t *nb_of_n, const int32_t *grid_per_bin,
+                              int32_t itmax, const int8_t *variant_of_n, const int32_t *grid_per_bin,
                               void *workspace, size_t workspace_bytes, veig_graph **out);
+/* Autotuner for batches like this one (no counterpart in the reference):
+ * times every kernel variant on the matrices of each order n <= 16 (lanes per
+ * matrix 4/8/16/32 as the widths allow), keeps the fastest per order, then
+ * times grids of 2/4/8/16 workgroups per CU for each variant in use.  Writes
+ * variant_of_n_out[33] and grid_per_bin_out[5] (host arrays) for
+ * veig_zgeev_varsize / veig_varsize_graph_create.  Runs the batch several
+ * times (its outputs are overwritten), synchronises the stream and allocates
+ * a little device scratch: an offline step, not for capture. */
+int veig_varsize_autotune(int64_t nmat, const int32_t *dims, const int64_t *a_off,
+                          const int64_t *e_off, const double *rA, const double *iA, double *rE,
+                          double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
+                          int8_t *variant_of_n_out, int32_t *grid_per_bin_out, void *workspace,
+                          size_t workspace_bytes, void *stream);
 int veig_graph_launch(veig_graph *graph, void *stream);
 int veig_graph_destroy(veig_graph *graph);
 

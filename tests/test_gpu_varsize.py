@@ -117,23 +117,46 @@ def test_varsize_graph_replays(gpu):
         assert err[0] <= 1e-11 * np.abs(ref[m][0]).max() * 2
 
 
-def test_varsize_nb_table(gpu):
-    # a different lane-group width per order (the autotuner's knob) gives the
-    # same eigenpairs
+def test_varsize_variant_table(gpu):
+    # other kernel variants per order (lane-group width and lanes per matrix,
+    # the autotuner's knob) give the same eigenpairs
     rng = np.random.default_rng(6)
     dims = rng.integers(2, 17, 200).tolist()
     vb, mats = build(dims, 7)
     vb.run()
     torch.cuda.synchronize()
     base = fetch(vb, dims)
-    table = np.zeros(33, np.int8)
-    table[1:17] = 32
-    vb.nb_table = torch.from_numpy(table).cuda()
+    for variants in ({n: 4 for n in range(1, 17)},                                  # groups of 32
+                     {**{n: 1 for n in range(1, 9)}, **{n: 3 for n in range(9, 17)}}):  # 2 indices per lane
+        table = np.full(33, -1, np.int8)
+        for n, v in variants.items():
+            table[n] = v
+        vb.variant_table = torch.from_numpy(table).cuda()
+        vb.grid = [256, 512, 768, 1024, 1280]
+        vb.rV.zero_()
+        vb.run()
+        torch.cuda.synchronize()
+        alt = fetch(vb, dims)
+        for m, n in enumerate(dims):
+            _, err = match_eigenvalues(base[m][0][None], alt[m][0][None])
+            assert err[0] < 1e-12 * max(1.0, np.abs(base[m][0]).max())
+            assert backward_error(mats[m][None], alt[m][0][None], alt[m][1][None])[0] < 1e-13
+
+
+def test_varsize_autotune(gpu):
+    rng = np.random.default_rng(8)
+    dims = rng.integers(1, 33, 600).tolist()
+    vb, mats = build(dims, 9)
+    tab, grid = vb.autotune()
+    assert len(tab) == 33 and len(grid) == 5
+    for n in range(1, 33):
+        v = tab[n]
+        width = 8 if v < 2 else (16 if v < 4 else 32)
+        assert 0 <= v <= 4 and width >= n, (n, v)
     vb.rV.zero_()
     vb.run()
     torch.cuda.synchronize()
-    alt = fetch(vb, dims)
+    got = fetch(vb, dims)
+    assert np.all(vb.info.cpu().numpy() == 0)
     for m, n in enumerate(dims):
-        _, err = match_eigenvalues(base[m][0][None], alt[m][0][None])
-        assert err[0] < 1e-12 * max(1.0, np.abs(base[m][0]).max())
-        assert backward_error(mats[m][None], alt[m][0][None], alt[m][1][None])[0] < 1e-13
+        assert backward_error(mats[m][None], got[m][0][None], got[m][1][None])[0] < 1e-13

@@ -11,9 +11,10 @@
 // whole sequence can be captured and replayed):
 //   k_bin_init     clears the size histogram (a fill kernel, not a memset)
 //   k_bin_count    histogram of the orders
-//   k_bin_scan     orders grouped by lane-group width NB (8, 16 or 32, the
-//                  per-order choice of `nb_of_n`), ordered by n inside a group:
-//                  start offsets and the three bin ranges
+//   k_bin_scan     orders grouped by kernel variant (the per-order choice of
+//                  `variant_of_n`: lane-group width NB and lanes per matrix),
+//                  ordered by n inside a variant: start offsets and the bin
+//                  ranges
 //   k_bin_scatter  permutation of matrix indices in that order
 //   k_zgeev_var<NB,P> one persistent kernel per bin: a fixed grid walks the bin's
 //                  range, read from device memory, wavefront by wavefront; the
@@ -22,6 +23,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "veig_common.hpp"
 
@@ -30,14 +32,19 @@ using namespace veig;
 namespace {
 
 constexpr int kMaxN = 32;
-constexpr int kBins = 3;  // NB = 8, 16, 32
+// kernel variants (bins): lane-group width NB and slots P, i.e. NB/P lanes
+// per matrix -- the threads-per-matrix choice of the autotuner
+constexpr int kBins = 5;
+__host__ __device__ constexpr int var_nb(int v) { return v < 2 ? 8 : (v < 4 ? 16 : 32); }
+__host__ __device__ constexpr int var_p(int v) { return (v == 1 || v == 3) ? 2 : 1; }
+__host__ __device__ constexpr int default_variant(int n) { return n <= 8 ? 0 : (n <= 16 ? 2 : 4); }
 
 struct BinState {
-  int32_t hist[kMaxN + 1];    // count per order
-  int32_t cursor[kMaxN + 1];  // scatter cursor per order
-  int64_t start[kBins];       // bin range in the permutation
+  int32_t hist[kMaxN + 1];     // count per order
+  int32_t cursor[kMaxN + 1];   // scatter cursor per order
+  int64_t start[kBins];        // bin range in the permutation
   int64_t count[kBins];
-  int32_t nb_of_n[kMaxN + 1]; // lane-group width per order
+  int32_t var_of_n[kMaxN + 1]; // kernel variant per order
 };
 
 struct VarArgs {
@@ -52,15 +59,15 @@ struct VarArgs {
   int itmax;
 };
 
-__global__ void k_bin_init(BinState *b, const int8_t *nb_table) {
+__global__ void k_bin_init(BinState *b, const int8_t *var_table) {
   const int i = threadIdx.x;
   if (i <= kMaxN) {
     b->hist[i] = 0;
     b->cursor[i] = 0;
-    int nb = nb_table ? nb_table[i] : 0;
-    if (nb != 8 && nb != 16 && nb != 32) nb = i <= 8 ? 8 : (i <= 16 ? 16 : 32);
-    if (nb < i) nb = i <= 8 ? 8 : (i <= 16 ? 16 : 32);  // a group must hold the matrix
-    b->nb_of_n[i] = nb;
+    int v = var_table ? var_table[i] : -1;
+    // unknown codes, and groups too narrow to hold the matrix: the default
+    if (v < 0 || v >= kBins || var_nb(v) < i) v = default_variant(i);
+    b->var_of_n[i] = v;
   }
   if (i < kBins) {
     b->start[i] = 0;
@@ -89,10 +96,9 @@ __global__ void k_bin_scan(BinState *b) {
   if (threadIdx.x != 0) return;
   int64_t pos = 0;
   for (int bi = 0; bi < kBins; ++bi) {
-    const int nb = 8 << bi;
     b->start[bi] = pos;
     for (int n = 1; n <= kMaxN; ++n) {
-      if (b->nb_of_n[n] != nb) continue;
+      if (b->var_of_n[n] != bi) continue;
       b->cursor[n] = (int32_t)pos;
       pos += b->hist[n];
     }
@@ -160,7 +166,7 @@ size_t varsize_ws_bytes(int64_t nmat) {
 
 int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, const int64_t *e_off,
                     const double *rA, const double *iA, double *rE, double *iE, double *rV, double *iV,
-                    int32_t *info, int32_t itmax, const int8_t *nb_table, const int32_t *grid_per_bin,
+                    int32_t *info, int32_t itmax, const int8_t *var_table, const int32_t *grid_per_bin,
                     void *workspace, hipStream_t s) {
   char *p = static_cast<char *>(workspace);
   VarArgs a;
@@ -183,25 +189,21 @@ int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, con
   VEIG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int64_t cgrid64 = (nmat + 255) / 256;
   const unsigned cgrid = (unsigned)(cgrid64 < 4 * (int64_t)cus ? (cgrid64 > 0 ? cgrid64 : 1) : 4 * cus);
-  hipLaunchKernelGGL(k_bin_init, dim3(1), dim3(64), 0, s, a.bins, nb_table);
+  hipLaunchKernelGGL(k_bin_init, dim3(1), dim3(64), 0, s, a.bins, var_table);
   hipLaunchKernelGGL(k_bin_count, dim3(cgrid), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
   hipLaunchKernelGGL(k_bin_scatter, dim3(cgrid), dim3(256), 0, s, a);
-  // persistent grids: a few resident waves per CU and bin (the three bins
-  // run back to back; every wave of a bin loops until the bin is drained)
-  const int defaults[kBins] = {8 * cus, 8 * cus, 4 * cus};
+  // persistent grids: a few resident waves per CU and bin (the bins run back
+  // to back; every wave of a bin loops until the bin is drained)
+  const int defaults[kBins] = {8 * cus, 8 * cus, 8 * cus, 4 * cus, 4 * cus};
   for (int b = 0; b < kBins; ++b) {
     const int g = (grid_per_bin && grid_per_bin[b] > 0) ? grid_per_bin[b] : defaults[b];
     switch (b) {
-      case 0:
-        if (slots_for(false, 8) == 2) hipLaunchKernelGGL((k_zgeev_var<8, 2>), dim3(g), dim3(64), 0, s, a, 0);
-        else hipLaunchKernelGGL((k_zgeev_var<8, 1>), dim3(g), dim3(64), 0, s, a, 0);
-        break;
-      case 1:
-        if (slots_for(false, 16) == 2) hipLaunchKernelGGL((k_zgeev_var<16, 2>), dim3(g), dim3(64), 0, s, a, 1);
-        else hipLaunchKernelGGL((k_zgeev_var<16, 1>), dim3(g), dim3(64), 0, s, a, 1);
-        break;
-      default: hipLaunchKernelGGL((k_zgeev_var<32, 1>), dim3(g), dim3(64), 0, s, a, 2); break;
+      case 0: hipLaunchKernelGGL((k_zgeev_var<8, 1>), dim3(g), dim3(64), 0, s, a, 0); break;
+      case 1: hipLaunchKernelGGL((k_zgeev_var<8, 2>), dim3(g), dim3(64), 0, s, a, 1); break;
+      case 2: hipLaunchKernelGGL((k_zgeev_var<16, 1>), dim3(g), dim3(64), 0, s, a, 2); break;
+      case 3: hipLaunchKernelGGL((k_zgeev_var<16, 2>), dim3(g), dim3(64), 0, s, a, 3); break;
+      default: hipLaunchKernelGGL((k_zgeev_var<32, 1>), dim3(g), dim3(64), 0, s, a, 4); break;
     }
   }
   VEIG_HIP(hipGetLastError());
@@ -224,7 +226,7 @@ size_t veig_varsize_workspace_size(int64_t nmat) {
 
 int veig_zgeev_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, const int64_t *e_off,
                        const double *rA, const double *iA, double *rE, double *iE, double *rV,
-                       double *iV, int32_t *info, int32_t itmax, const int8_t *nb_of_n,
+                       double *iV, int32_t *info, int32_t itmax, const int8_t *variant_of_n,
                        const int32_t *grid_per_bin, void *workspace, size_t workspace_bytes,
                        void *stream) {
   if (nmat < 0) return fail(VEIG_ERR_INVALID, "nmat < 0");
@@ -236,14 +238,14 @@ int veig_zgeev_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, 
     return fail(VEIG_ERR_INVALID, "workspace too small: need " + std::to_string(varsize_ws_bytes(nmat)) + " bytes");
   if (reinterpret_cast<uintptr_t>(workspace) % kAlign != 0)
     return fail(VEIG_ERR_INVALID, "workspace must be 256-byte aligned");
-  return enqueue_varsize(nmat, dims, a_off, e_off, rA, iA, rE, iE, rV, iV, info, itmax, nb_of_n,
+  return enqueue_varsize(nmat, dims, a_off, e_off, rA, iA, rE, iE, rV, iV, info, itmax, variant_of_n,
                          grid_per_bin, workspace, (hipStream_t)stream);
 }
 
 int veig_varsize_graph_create(int64_t nmat, const int32_t *dims, const int64_t *a_off,
                               const int64_t *e_off, const double *rA, const double *iA, double *rE,
                               double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
-                              const int8_t *nb_of_n, const int32_t *grid_per_bin, void *workspace,
+                              const int8_t *variant_of_n, const int32_t *grid_per_bin, void *workspace,
                               size_t workspace_bytes, veig_graph **out) {
   VEIG_REQUIRE(out);
   *out = nullptr;
@@ -255,7 +257,7 @@ int veig_varsize_graph_create(int64_t nmat, const int32_t *dims, const int64_t *
     return hip_fail(e, "hipStreamBeginCapture");
   }
   const int st = veig_zgeev_varsize(nmat, dims, a_off, e_off, rA, iA, rE, iE, rV, iV, info, itmax,
-                                    nb_of_n, grid_per_bin, workspace, workspace_bytes, cs);
+                                    variant_of_n, grid_per_bin, workspace, workspace_bytes, cs);
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(cs, &graph);
   (void)hipStreamDestroy(cs);
@@ -274,6 +276,115 @@ int veig_varsize_graph_create(int64_t nmat, const int32_t *dims, const int64_t *
   }
   *out = g;
   return VEIG_OK;
+}
+
+int veig_varsize_autotune(int64_t nmat, const int32_t *dims, const int64_t *a_off, const int64_t *e_off,
+                          const double *rA, const double *iA, double *rE, double *iE, double *rV,
+                          double *iV, int32_t *info, int32_t itmax, int8_t *variant_of_n_out,
+                          int32_t *grid_per_bin_out, void *workspace, size_t workspace_bytes,
+                          void *stream) {
+  VEIG_REQUIRE(variant_of_n_out);
+  VEIG_REQUIRE(grid_per_bin_out);
+  for (int n = 0; n <= kMaxN; ++n) variant_of_n_out[n] = (int8_t)default_variant(n);
+  for (int b = 0; b < kBins; ++b) grid_per_bin_out[b] = 0;
+  if (nmat <= 0) return nmat < 0 ? fail(VEIG_ERR_INVALID, "nmat < 0") : VEIG_OK;
+  VEIG_REQUIRE(dims);
+  if (workspace_bytes < varsize_ws_bytes(nmat))
+    return fail(VEIG_ERR_INVALID, "workspace too small: need " + std::to_string(varsize_ws_bytes(nmat)) + " bytes");
+  hipStream_t s = (hipStream_t)stream;
+  // orders present (host copy of dims: the tuner is an offline step)
+  std::vector<int32_t> hd((size_t)nmat);
+  VEIG_HIP(hipMemcpyAsync(hd.data(), dims, (size_t)nmat * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  VEIG_HIP(hipStreamSynchronize(s));
+  int64_t hist[kMaxN + 1] = {0};
+  for (int32_t n : hd)
+    if (n >= 1 && n <= kMaxN) ++hist[n];
+  // device scratch: a dims copy restricted to one order, and a variant table
+  int32_t *dsub = nullptr;
+  int8_t *dtab = nullptr;
+  VEIG_HIP(hipMalloc(&dsub, (size_t)nmat * sizeof(int32_t)));
+  if (hipMalloc(&dtab, kMaxN + 1) != hipSuccess) {
+    (void)hipFree(dsub);
+    return fail(VEIG_ERR_NOMEM, "hipMalloc (tuner table)");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int st = VEIG_OK;
+  auto time_run = [&](const int32_t *d, const int8_t *tab, const int32_t *grid, float &ms) -> int {
+    // one untimed run, then the best of two timed runs
+    ms = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      if (hipEventRecord(e0, s) != hipSuccess) return fail(VEIG_ERR_HIP, "hipEventRecord");
+      const int r = enqueue_varsize(nmat, d, a_off, e_off, rA, iA, rE, iE, rV, iV, info, itmax, tab, grid,
+                                    workspace, s);
+      if (r != VEIG_OK) return r;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+        return fail(VEIG_ERR_HIP, "hipEventSynchronize");
+      float t = 0;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      if (rep > 0 && t < ms) ms = t;
+    }
+    return VEIG_OK;
+  };
+  // 1. per order, the fastest variant (matrices of the other orders masked to n = 0)
+  int8_t tab[kMaxN + 1];
+  std::vector<int32_t> sub((size_t)nmat);
+  for (int n = 1; n <= kMaxN && st == VEIG_OK; ++n) {
+    if (!hist[n] || n > 16) continue;  // orders above 16 have a single variant
+    for (size_t m = 0; m < (size_t)nmat; ++m) sub[m] = hd[m] == n ? n : 0;
+    if (hipMemcpy(dsub, sub.data(), (size_t)nmat * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+      st = fail(VEIG_ERR_HIP, "hipMemcpy (tuner dims)");
+      break;
+    }
+    float best = 1e30f;
+    for (int v = 0; v < kBins && st == VEIG_OK; ++v) {
+      if (var_nb(v) < n) continue;
+      for (int k = 0; k <= kMaxN; ++k) tab[k] = (int8_t)default_variant(k);
+      tab[n] = (int8_t)v;
+      if (hipMemcpy(dtab, tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess) {
+        st = fail(VEIG_ERR_HIP, "hipMemcpy (tuner table)");
+        break;
+      }
+      float ms = 0;
+      st = time_run(dsub, dtab, nullptr, ms);
+      if (st == VEIG_OK && ms < best) {
+        best = ms;
+        variant_of_n_out[n] = (int8_t)v;
+      }
+    }
+  }
+  // 2. per bin, the persistent grid (bins run back to back: their times add)
+  if (st == VEIG_OK && hipMemcpy(dtab, variant_of_n_out, kMaxN + 1, hipMemcpyHostToDevice) != hipSuccess)
+    st = fail(VEIG_ERR_HIP, "hipMemcpy (tuner table)");
+  int dev = 0, cus = 256;
+  if (st == VEIG_OK && (hipGetDevice(&dev) != hipSuccess ||
+                        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess))
+    st = fail(VEIG_ERR_HIP, "hipDeviceGetAttribute");
+  for (int b = 0; b < kBins && st == VEIG_OK; ++b) {
+    bool used = false;
+    for (int n = 1; n <= kMaxN; ++n) used |= hist[n] && variant_of_n_out[n] == b;
+    if (!used) continue;
+    float best = 1e30f;
+    const int cand[4] = {2, 4, 8, 16};
+    for (int c : cand) {
+      int32_t grid[kBins] = {0};
+      for (int k = 0; k < kBins; ++k) grid[k] = grid_per_bin_out[k];
+      grid[b] = c * cus;
+      float ms = 0;
+      st = time_run(dims, dtab, grid, ms);
+      if (st != VEIG_OK) break;
+      if (ms < best) {
+        best = ms;
+        grid_per_bin_out[b] = c * cus;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(dsub);
+  (void)hipFree(dtab);
+  return st;
 }
 
 int veig_graph_launch(veig_graph *g, void *stream) {

@@ -73,6 +73,9 @@ _SIGS = {
     "veig_varsize_graph_create": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                    ctypes.c_int32, _vp, _vp, _vp, ctypes.c_size_t,
                                    ctypes.POINTER(ctypes.c_void_p)], _i),
+    "veig_varsize_autotune": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32,
+                               ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(ctypes.c_int32), _vp,
+                               ctypes.c_size_t, _vp], _i),
     "veig_graph_launch": ([_vp, _vp], _i),
     "veig_graph_destroy": ([_vp], _i),
 }
@@ -265,13 +268,17 @@ class VarBatch:
         self.info = torch.zeros(self.nmat, dtype=torch.int32, device=dev)
         nbytes = lib().veig_varsize_workspace_size(self.nmat)
         self.ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
-        self.nb_table = None
-        self.grid = None
+        self.variant_table = None  # device int8[33]: kernel variant per order (None: defaults)
+        self.grid = None           # 5 ints: persistent grid per variant (None / 0: defaults)
 
     def _args(self, itmax):
+        grid = None
+        if self.grid is not None:   # host array, read when the launches are enqueued
+            self._grid_host = (ctypes.c_int32 * 5)(*[int(x) for x in self.grid])
+            grid = ctypes.cast(self._grid_host, ctypes.c_void_p)
         return (self.nmat, _ptr(self.dims), _ptr(self.a_off), _ptr(self.e_off), _ptr(self.rA),
                 _ptr(self.iA), _ptr(self.rE), _ptr(self.iE), _ptr(self.rV), _ptr(self.iV),
-                _ptr(self.info), itmax, _ptr(self.nb_table), _ptr(self.grid), _ptr(self.ws),
+                _ptr(self.info), itmax, _ptr(self.variant_table), grid, _ptr(self.ws),
                 self.ws.numel())
 
     def run(self, itmax: int = 30, stream=None):
@@ -283,6 +290,20 @@ class VarBatch:
         check(lib().veig_varsize_graph_create(*self._args(itmax), ctypes.byref(h)),
               "veig_varsize_graph_create")
         return Graph(h)
+
+    def autotune(self, itmax: int = 30, stream=None):
+        """Run the library's autotuner on this batch (overwrites the outputs)
+        and keep its choice for run()/graph(); returns (variant_of_n, grid)."""
+        torch = _torch()
+        tab = (ctypes.c_int8 * 33)()
+        grid = (ctypes.c_int32 * 5)()
+        args = self._args(itmax)[:12]
+        check(lib().veig_varsize_autotune(*args, tab, grid, _ptr(self.ws), self.ws.numel(), _stream(stream)),
+              "veig_varsize_autotune")
+        dev = self.dims.device
+        self.variant_table = torch.tensor(list(tab), dtype=torch.int8, device=dev)
+        self.grid = list(grid)
+        return list(tab), list(grid)
 
 
 class Graph:
