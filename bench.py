@@ -175,7 +175,8 @@ def bench_uniform(a, cfg, veig, W, torch, dev, stream):
         rf = roofline(flops, nbytes, mixed["fp32_kernel_ms"], f"k_zgeev<float, {nb(n)}, 1, 0>", f"zgeev_fp32_n{n}")
     else:
         value, ms, dtype = v64, ms64, "f64"
-        rf = roofline(flops, nbytes, ms64, f"k_zgeev<double, {nb(n)}, 1, 0>", f"zgeev_fp64_n{n}")
+        kname = "k_zgeev64 (two wavefronts per matrix)" if n > 32 else f"k_zgeev<double, {nb(n)}, 1, 0>"
+        rf = roofline(flops, nbytes, ms64, kname, f"zgeev_fp64_n{n}")
     rf["flops_per_matrix"] = nominal_flops(n)
 
     if not a.no_cpu:

@@ -42,7 +42,7 @@ def test_order_one_and_two(gpu):
     assert err.max() < 1e-13 and np.max(H.backward_error(A2, E, V)) < 1e-14
 
 
-@pytest.mark.parametrize("n", [5, 16, 32])
+@pytest.mark.parametrize("n", [5, 16, 32, 48, 64])
 def test_structured_inputs(gpu, n):
     rng = np.random.default_rng(n)
     eye = np.broadcast_to(np.eye(n, dtype=complex), (8, n, n)).copy()

@@ -150,3 +150,33 @@ def test_order_64_tiles(gpu, thread_cnt):
     Eo, Vo, _ = oracle_eig(A, thread_cnt)
     E, V, info = gpu_eig(A, thread_cnt)
     check_parity(A, E, V, info, Eo, Vo)
+
+
+def unit_upper_eigvecs(T):
+    """X with column k = [x; 1; 0], (T - T_kk I) X(:, k) = 0 (numpy solve)."""
+    N, n, _ = T.shape
+    X = np.zeros_like(T)
+    for b in range(N):
+        for k in range(n):
+            X[b, k, k] = 1
+            if k:
+                M = T[b, :k, :k] - T[b, k, k] * np.eye(k)
+                X[b, :k, k] = np.linalg.solve(M, -T[b, :k, k])
+    return X
+
+
+@pytest.mark.parametrize("n", [16, 40, 64])
+def test_schur_form_written(gpu, n):
+    # VEIG_WRITE_SCHUR: the input planes receive T; Z = V X^{-1} must be
+    # unitary with A Z = Z T, T upper triangular with diag(T) = E
+    from gpu_util import gpu_eig
+    A = gaussian(8, n, 5000 + n)
+    E, V, info, T = gpu_eig(A, 4, write_schur=True)
+    assert np.all(info == 0)
+    assert np.all(np.tril(T, -1) == 0)
+    assert np.array_equal(np.diagonal(T, 0, 1, 2), E)
+    Z = V @ np.linalg.inv(unit_upper_eigvecs(T))
+    eye = np.eye(n)
+    nrm = np.linalg.norm(A, axis=(1, 2))
+    assert np.max(np.linalg.norm(np.conj(np.transpose(Z, (0, 2, 1))) @ Z - eye, axis=(1, 2))) < 1e-12 * n
+    assert np.max(np.linalg.norm(A @ Z - Z @ T, axis=(1, 2)) / nrm) < 1e-13 * n

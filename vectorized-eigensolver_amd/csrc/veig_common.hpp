@@ -385,6 +385,9 @@ inline void launch_one(const FusedArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((k_zgeev<R, NB, P, OUT>), dim3((unsigned)grid), dim3(64), 0, s, a);
 }
 
+// orders 33..64 in FP64, one matrix per two-wavefront workgroup (whole batches)
+int launch_zgeev64(const FusedArgs &a, hipStream_t s);
+
 template <typename R, int OUT = OUT_V>
 int launch_zgeev(const FusedArgs &a, hipStream_t s) {
   constexpr bool f32 = std::is_same<R, float>::value;
@@ -416,6 +419,9 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
       launch_one<R, 32, 1, OUT>(a, s);
       break;
     default:
+      if constexpr (std::is_same<R, double>::value && OUT == OUT_V) {
+        if (!a.list && !a.mask) return launch_zgeev64(a, s);
+      }
       if constexpr (OUT == OUT_V) {
         launch_one<R, 64, 1, OUT_V>(a, s);
       } else {
