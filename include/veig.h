@@ -43,8 +43,8 @@ typedef enum {
 /* flags for veig_zgeev */
 #define VEIG_WRITE_SCHUR 1u /* also store the Schur form T over A (zeros below the diagonal) */
 
-/* Largest order of the fused FP64/complex64 entries (veig_zgeev,
- * veig_zgeev_fp32, veig_host_eigensolver) and of the other entries. */
+/* Largest order of the fused entries (veig_zgeev, veig_zgeev_fp32,
+ * veig_zgeev_mixed, veig_host_eigensolver) and of the other entries. */
 #define VEIG_MAX_DIM 64
 #define VEIG_MAX_DIM_STAGES 32
 
@@ -77,8 +77,8 @@ int veig_zgeev_fp32(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const do
 
 /* ---------------------------------------------------------------------------
  * Mixed precision.  No counterpart in the reference (new in this design, for
- * the same eigendecomposition as veig_zgeev): the reduction, QR sweeps and
- * eigenvector solves run in complex64; every eigenpair is then refined in
+ * the same eigendecomposition as veig_zgeev, orders up to VEIG_MAX_DIM): the
+ * reduction, QR sweeps and eigenvector solves run in complex64; every eigenpair is then refined in
  * FP64 by Newton's method on R = A V - V diag(E) until the backward error
  * ||R||_F / (||A||_F ||V||_F) <= tol (at most 3 corrections); matrices that do
  * not get there, or whose complex64 QR did not converge, are solved again by

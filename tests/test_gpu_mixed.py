@@ -48,7 +48,7 @@ def check(A, E, V, info, fb, be, tol, cond=1.0):
     return my_be
 
 
-@pytest.mark.parametrize("n", [4, 8, 16, 24, 32])
+@pytest.mark.parametrize("n", [4, 8, 16, 24, 32, 40, 64])
 def test_mixed_gaussian(gpu, n):
     A = gaussian(128, n, 5000 + n)
     E, V, info, fb, be = run_mixed(A)
@@ -63,12 +63,14 @@ def test_mixed_hermitian(gpu, n):
     check(A, E, V, info, fb, be, 1e-12)
 
 
-@pytest.mark.parametrize("n,cond", [(24, 1e1), (24, 1e3), (24, 1e6), (16, 1e3), (16, 1e6), (8, 1e6)])
+@pytest.mark.parametrize("n,cond", [(24, 1e1), (24, 1e3), (24, 1e6), (16, 1e3), (16, 1e6), (8, 1e6),
+                                    (64, 1e3), (48, 1e6)])
 def test_mixed_conditioned(gpu, n, cond):
     # configuration of the mixed-precision workload: A = V diag(lam) V^-1
     # with cond(V) up to 1e6; matrices the refinement cannot bring to the
     # tolerance fall back to FP64, so the bar holds for all of them (n <= 16
-    # refines from the complex64 Schur factors, n > 16 from V and W)
+    # refines from the complex64 Schur factors, n > 16 from V and W; n > 32
+    # on four wavefronts per matrix)
     A = constructed(128, n, 5200 + 10 * n + int(np.log10(cond)), cond=cond)
     E, V, info, fb, be = run_mixed(A)
     check(A, E, V, info, fb, be, 1e-12, cond=cond)
@@ -81,14 +83,37 @@ def test_mixed_tolerance_and_tiles(gpu):
         check(A, E, V, info, fb, be, 1e-14)
 
 
-def test_fp32_solve_accuracy(gpu):
+@pytest.mark.parametrize("n", [16, 64])
+def test_fp32_solve_accuracy(gpu, n):
     # the complex64 building block alone: single-precision accuracy
     from gpu_util import gpu_eig
-    A = gaussian(128, 16, 5400)
+    A = gaussian(128, n, 5400 + n)
     E, V, info = gpu_eig(A, 1, precision="fp32")
     be = backward_error(A, E, V)
     assert np.all(info == 0)
     assert be.max() < 1e-5 and be.max() > 1e-9, f"fp32 backward error {be.max():.2e}"
+
+
+def test_mixed_large_order_tiles(gpu):
+    A = gaussian(64, 64, 5500)
+    for tc in (1, 4):
+        E, V, info, fb, be = run_mixed(A, tc=tc, tol=1e-13)
+        check(A, E, V, info, fb, be, 1e-13)
+
+
+@pytest.mark.parametrize("n", [16, 40])
+def test_mixed_all_fall_back(gpu, n):
+    # an unreachable tolerance sends every matrix to the FP64 solve of the
+    # list (and its backward-error pass): FP64 results, flagged, info 0
+    A = gaussian(64, n, 5600 + n)
+    E, V, info, fb, be = run_mixed(A, tol=1e-300)
+    assert np.all(fb == 1) and np.all(info == 0)
+    my_be = backward_error(A, E, V)
+    assert my_be.max() < 64 * n * np.finfo(np.float64).eps
+    assert np.allclose(be, my_be, rtol=1e-3, atol=1e-16)
+    Eo, _, _ = oracle_eig(A, 1)
+    _, err = match_eigenvalues(Eo, E)
+    assert np.all(err <= 1e-11 * np.linalg.norm(A, axis=(1, 2)))
 
 
 def test_mixed_workspace_checks(gpu):

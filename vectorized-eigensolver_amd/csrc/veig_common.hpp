@@ -385,7 +385,9 @@ inline void launch_one(const FusedArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((k_zgeev<R, NB, P, OUT>), dim3((unsigned)grid), dim3(64), 0, s, a);
 }
 
-// orders 33..64 in FP64, one matrix per two-wavefront workgroup (whole batches)
+// orders 33..64, one matrix per two-wavefront workgroup (instantiated for
+// <double, OUT_V>, <float, OUT_V> and <float, OUT_VW>)
+template <typename R, int OUT>
 int launch_zgeev64(const FusedArgs &a, hipStream_t s);
 
 template <typename R, int OUT = OUT_V>
@@ -419,15 +421,11 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
       launch_one<R, 32, 1, OUT>(a, s);
       break;
     default:
-      if constexpr (std::is_same<R, double>::value && OUT == OUT_V) {
-        if (!a.list && !a.mask) return launch_zgeev64(a, s);
-      }
-      if constexpr (OUT == OUT_V) {
-        launch_one<R, 64, 1, OUT_V>(a, s);
+      if constexpr (OUT == OUT_V || (f32 && OUT == OUT_VW)) {
+        return launch_zgeev64<R, OUT>(a, s);
       } else {
-        return fail(VEIG_ERR_UNSUPPORTED, "mixed precision handles orders up to 32");
+        return fail(VEIG_ERR_UNSUPPORTED, "this output form handles orders up to 32");
       }
-      break;
   }
   VEIG_HIP(hipGetLastError());
   return VEIG_OK;

@@ -30,29 +30,33 @@ constexpr int kN = 64;
 // One QR sweep as the Schur-vector engine needs it: the bulge-chase
 // reflectors for rows K = L..I-1 (t1, v2 of each) and, if has_phase, the
 // phase that made H(I, I-1) real, applied to column I.
+template <typename R>
 struct SweepLog {
-  double2 t1[kN];
-  double2 v2[kN];
-  double2 phase;
+  using L2 = typename lds2<R>::type;
+  L2 t1[kN];
+  L2 v2[kN];
+  L2 phase;
   int32_t L, I, has_phase, last;
 };
 
+template <typename R>
 struct LargeShared {
-  double2 H[kN * (kN + 1)];
-  double2 v[2][kN];   // Hessenberg reflectors (double-buffered)
-  double2 tau[2];
-  SweepLog log[2];    // sweeps (double-buffered)
+  using L2 = typename lds2<R>::type;
+  L2 H[kN * (kN + 1)];
+  L2 v[2][kN];         // Hessenberg reflectors (double-buffered)
+  L2 tau[2];
+  SweepLog<R> log[2];  // sweeps (double-buffered)
 };
-
-using Gr = Group<double, kN, 1>;
 
 // The Hessenberg engine's QR iteration (lahqr with one matrix per wavefront),
 // recording the Schur-vector updates instead of applying them.  Ends every
 // sweep (and the iteration) at a workgroup barrier; the last one is flagged.
-__device__ int lahqr_recorded(const Gr &g, LargeShared &sh, int itmax) {
+template <typename R>
+__device__ int lahqr_recorded(const Group<R, kN, 1> &g, LargeShared<R> &sh, int itmax) {
+  using L2 = typename lds2<R>::type;
   const int r = g.r, n = g.n;
-  const double ulp = limits<double>::eps;
-  const double smlnum = limits<double>::tiny * (double(n > 0 ? n : 1) / ulp);
+  const R ulp = limits<R>::eps;
+  const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
   int I = n - 1, L = 0, its = 0, nfail = 0;
   bool done = (n <= 0);
   int buf = 0;
@@ -63,22 +67,22 @@ __device__ int lahqr_recorded(const Gr &g, LargeShared &sh, int itmax) {
       bool neg = false;
       const int k = r;
       if (its <= itmax && k > L && k <= I) {
-        const cx<double> h = ld<double>(g.at(k, k - 1));
+        const cx<R> h = ld<R>(g.at(k, k - 1));
         if (cabs1(h) <= smlnum) {
           neg = true;
         } else {
-          const cx<double> hkk1 = ld<double>(g.at(k - 1, k - 1)), hkk = ld<double>(g.at(k, k));
-          double tst = cabs1(hkk1) + cabs1(hkk);
-          if (tst == 0.0) {
-            if (k - 2 >= 0) tst += fabs(ld<double>(g.at(k - 1, k - 2)).re);
-            if (k + 1 <= n - 1) tst += fabs(ld<double>(g.at(k + 1, k)).re);
+          const cx<R> hkk1 = ld<R>(g.at(k - 1, k - 1)), hkk = ld<R>(g.at(k, k));
+          R tst = cabs1(hkk1) + cabs1(hkk);
+          if (tst == R(0)) {
+            if (k - 2 >= 0) tst += fabs(ld<R>(g.at(k - 1, k - 2)).re);
+            if (k + 1 <= n - 1) tst += fabs(ld<R>(g.at(k + 1, k)).re);
           }
           if (fabs(h.re) <= ulp * tst) {
-            const cx<double> hup = ld<double>(g.at(k - 1, k));
-            const double ab = fmax(cabs1(h), cabs1(hup)), ba = fmin(cabs1(h), cabs1(hup));
-            const double d3 = cabs1(sub(hkk1, hkk));
-            const double aa = fmax(cabs1(hkk), d3), bb = fmin(cabs1(hkk), d3);
-            const double s = aa + ab;
+            const cx<R> hup = ld<R>(g.at(k - 1, k));
+            const R ab = fmax(cabs1(h), cabs1(hup)), ba = fmin(cabs1(h), cabs1(hup));
+            const R d3 = cabs1(sub(hkk1, hkk));
+            const R aa = fmax(cabs1(hkk), d3), bb = fmin(cabs1(hkk), d3);
+            const R s = aa + ab;
             neg = ba * (ab / s) <= fmax(smlnum, ulp * (bb * (aa / s)));
           }
         }
@@ -92,7 +96,7 @@ __device__ int lahqr_recorded(const Gr &g, LargeShared &sh, int itmax) {
         if (I < 0) done = true;
       } else {
         if (bits) L = 63 - __clzll(bits);
-        if (L > 0 && r == 0) st<double>(g.at(L, L - 1), mk<double>(0, 0));
+        if (L > 0 && r == 0) st<R>(g.at(L, L - 1), mk<R>(0, 0));
         if (L >= I) {
           --I;
           L = 0;
@@ -107,94 +111,94 @@ __device__ int lahqr_recorded(const Gr &g, LargeShared &sh, int itmax) {
     if (!ready) break;
 
     // ---- shift and first column (Toolbox.hpp:896-1015) ----
-    cx<double> t;
+    cx<R> t;
     if (its == 10) {
-      const double s = 0.75 * fabs(ld<double>(g.at(L + 1, L)).re);
-      t = ld<double>(g.at(L, L));
+      const R s = R(0.75) * fabs(ld<R>(g.at(L + 1, L)).re);
+      t = ld<R>(g.at(L, L));
       t.re = s + t.re;
     } else if (its == 20) {
-      const double s = 0.75 * fabs(ld<double>(g.at(I, I - 1)).re);
-      t = ld<double>(g.at(I, I));
+      const R s = R(0.75) * fabs(ld<R>(g.at(I, I - 1)).re);
+      t = ld<R>(g.at(I, I));
       t.re = s + t.re;
     } else {
-      t = ld<double>(g.at(I, I));
-      cx<double> u = mul(csqrt(ld<double>(g.at(I - 1, I))), csqrt(ld<double>(g.at(I, I - 1))));
-      double s = cabs1(u);
-      if (s != 0.0) {
-        cx<double> x = scale(sub(ld<double>(g.at(I - 1, I - 1)), t), 0.5);
-        const double sx = cabs1(x);
+      t = ld<R>(g.at(I, I));
+      cx<R> u = mul(csqrt(ld<R>(g.at(I - 1, I))), csqrt(ld<R>(g.at(I, I - 1))));
+      R s = cabs1(u);
+      if (s != R(0)) {
+        cx<R> x = scale(sub(ld<R>(g.at(I - 1, I - 1)), t), R(0.5));
+        const R sx = cabs1(x);
         s = fmax(s, sx);
-        const double is = frcp(s);
-        const cx<double> xs = scale(x, is), us = scale(u, is);
-        cx<double> y = scale(csqrt(add(mul(xs, xs), mul(us, us))), s);
-        if (sx > 0.0 && x.re * y.re + x.im * y.im < 0.0) y = mk(-y.re, -y.im);
+        const R is = frcp(s);
+        const cx<R> xs = scale(x, is), us = scale(u, is);
+        cx<R> y = scale(csqrt(add(mul(xs, xs), mul(us, us))), s);
+        if (sx > R(0) && x.re * y.re + x.im * y.im < R(0)) y = mk(-y.re, -y.im);
         t = sub(t, mul(cdiv(u, add(x, y)), u));
       }
     }
-    const cx<double> h11s = sub(ld<double>(g.at(L, L)), t);
-    const double h21 = ld<double>(g.at(L + 1, L)).re;
-    const double s0 = cabs1(h11s) + fabs(h21);
-    const double is0 = frcp(s0);
-    const cx<double> a0 = scale(h11s, is0), a1 = mk<double>(h21 * is0, 0);
+    const cx<R> h11s = sub(ld<R>(g.at(L, L)), t);
+    const R h21 = ld<R>(g.at(L + 1, L)).re;
+    const R s0 = cabs1(h11s) + fabs(h21);
+    const R is0 = frcp(s0);
+    const cx<R> a0 = scale(h11s, is0), a1 = mk<R>(h21 * is0, 0);
 
     // ---- the sweep (Toolbox.hpp:138-335), rotations recorded ----
-    SweepLog &lg = sh.log[buf];
+    SweepLog<R> &lg = sh.log[buf];
     for (int K = L; K < I; ++K) {
-      cx<double> b0 = a0, b1 = a1;
+      cx<R> b0 = a0, b1 = a1;
       if (K > L) {
-        b0 = ld<double>(g.at(K, K - 1));
-        b1 = ld<double>(g.at(K + 1, K - 1));
+        b0 = ld<R>(g.at(K, K - 1));
+        b1 = ld<R>(g.at(K + 1, K - 1));
       }
-      cx<double> t1, v2, nb0;
+      cx<R> t1, v2, nb0;
       larfg2(b0, b1, t1, v2, nb0);
-      const double t2 = t1.re * v2.re - t1.im * v2.im;  // Re(t1 * v2)
+      const R t2 = t1.re * v2.re - t1.im * v2.im;  // Re(t1 * v2)
       if (r == 0) {
         if (K > L) {
-          st<double>(g.at(K, K - 1), nb0);
-          st<double>(g.at(K + 1, K - 1), mk<double>(0, 0));
+          st<R>(g.at(K, K - 1), nb0);
+          st<R>(g.at(K + 1, K - 1), mk<R>(0, 0));
         }
-        lg.t1[K] = double2{t1.re, t1.im};
-        lg.v2[K] = double2{v2.re, v2.im};
+        st<R>(&lg.t1[K], t1);
+        st<R>(&lg.v2[K], v2);
       }
       if (r >= K && r < n) {  // rows K, K+1 over columns K..n-1
-        double2 *p1 = g.at(K, r);
-        double2 *p2 = g.at(K + 1, r);
-        cx<double> h1 = ld<double>(p1), h2 = ld<double>(p2);
-        const cx<double> sum = add(cmul(t1, h1), scale(h2, t2));
+        L2 *p1 = g.at(K, r);
+        L2 *p2 = g.at(K + 1, r);
+        cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+        const cx<R> sum = add(cmul(t1, h1), scale(h2, t2));
         h1 = sub(h1, sum);
         h2 = sub(h2, mul(sum, v2));
-        st<double>(p1, h1);
-        st<double>(p2, h2);
+        st<R>(p1, h1);
+        st<R>(p2, h2);
       }
       wave_sync();
       const int i3 = (K + 2) < I ? (K + 2) : I;
       if (r <= i3) {  // columns K, K+1 over rows 0..min(K+2, I)
-        double2 *p1 = g.at(r, K);
-        double2 *p2 = g.at(r, K + 1);
-        cx<double> h1 = ld<double>(p1), h2 = ld<double>(p2);
-        const cx<double> sum = add(mul(t1, h1), scale(h2, t2));
+        L2 *p1 = g.at(r, K);
+        L2 *p2 = g.at(r, K + 1);
+        cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+        const cx<R> sum = add(mul(t1, h1), scale(h2, t2));
         h1 = sub(h1, sum);
         h2 = sub(h2, mulc(sum, v2));
-        st<double>(p1, h1);
-        st<double>(p2, h2);
+        st<R>(p1, h1);
+        st<R>(p2, h2);
       }
       wave_sync();
     }
     // ---- make H(I, I-1) real (Toolbox.hpp:337-367) ----
     bool has_phase = false;
     {
-      const cx<double> h = ld<double>(g.at(I, I - 1));
-      if (h.im != 0.0 && abs2(h) > 0.0) {
+      const cx<R> h = ld<R>(g.at(I, I - 1));
+      if (h.im != R(0) && abs2(h) > R(0)) {
         has_phase = true;
-        const double id = frsq(abs2(h));
-        const double d = abs2(h) * id;
-        const cx<double> zz = mk(h.re * id, -h.im * id);  // conj(h/|h|)
+        const R id = frsq(abs2(h));
+        const R d = abs2(h) * id;
+        const cx<R> zz = mk(h.re * id, -h.im * id);  // conj(h/|h|)
         if (r == 0) {
-          st<double>(g.at(I, I - 1), mk<double>(d, 0));
-          lg.phase = double2{zz.re, zz.im};
+          st<R>(g.at(I, I - 1), mk<R>(d, 0));
+          st<R>(&lg.phase, zz);
         }
-        if (r > I && r < n) st<double>(g.at(I, r), mul(ld<double>(g.at(I, r)), zz));
-        if (r < I) st<double>(g.at(r, I), mulc(ld<double>(g.at(r, I)), zz));
+        if (r > I && r < n) st<R>(g.at(I, r), mul(ld<R>(g.at(I, r)), zz));
+        if (r < I) st<R>(g.at(r, I), mulc(ld<R>(g.at(r, I)), zz));
       }
     }
     ++its;
@@ -222,24 +226,25 @@ __device__ int lahqr_recorded(const Gr &g, LargeShared &sh, int itmax) {
 // A sweep's reflectors act on consecutive column pairs (K, K+1), K = L..I-1,
 // so the replay is a fully unrolled pass over K with a wavefront-uniform
 // guard: every z[] index is a compile-time constant and Z stays in registers.
-__device__ void replay_sweeps(const LargeShared &sh, cx<double> (&z)[kN]) {
+template <typename R>
+__device__ void replay_sweeps(const LargeShared<R> &sh, cx<R> (&z)[kN]) {
   int buf = 0;
   for (;;) {
     __syncthreads();
-    const SweepLog &lg = sh.log[buf];
+    const SweepLog<R> &lg = sh.log[buf];
     const int L = lg.L, I = lg.I;
 #pragma unroll
     for (int K = 0; K + 1 < kN; ++K) {
       if (K >= L && K < I) {
-        const cx<double> t1 = ld<double>(&lg.t1[K]), v2 = ld<double>(&lg.v2[K]);
-        const double t2 = t1.re * v2.re - t1.im * v2.im;
-        const cx<double> sum = add(mul(t1, z[K]), scale(z[K + 1], t2));
+        const cx<R> t1 = ld<R>(&lg.t1[K]), v2 = ld<R>(&lg.v2[K]);
+        const R t2 = t1.re * v2.re - t1.im * v2.im;
+        const cx<R> sum = add(mul(t1, z[K]), scale(z[K + 1], t2));
         z[K] = sub(z[K], sum);
         z[K + 1] = sub(z[K + 1], mulc(sum, v2));
       }
     }
     if (lg.has_phase) {
-      const cx<double> ph = ld<double>(&lg.phase);
+      const cx<R> ph = ld<R>(&lg.phase);
 #pragma unroll
       for (int c = 1; c < kN; ++c)
         if (c == I) z[c] = mulc(z[c], ph);
@@ -250,14 +255,85 @@ __device__ void replay_sweeps(const LargeShared &sh, cx<double> (&z)[kN]) {
   }
 }
 
-__global__ __launch_bounds__(128) void k_zgeev64(FusedArgs a) {
-  __shared__ LargeShared sh;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t m = xcd_block();  // one matrix per workgroup
-  if (m >= a.G.nmat) return;      // workgroup-uniform
+// Hessenberg engine, reduction step i: the reflector annihilating
+// H(i+2:n, i) (zlarfg, LAPACK sign) is published to the Schur-vector engine
+// at a workgroup barrier, then applied to H from both sides.
+template <typename R>
+__device__ void gehrd_step(const Group<R, kN, 1> &g, LargeShared<R> &sh, int i) {
+  using L2 = typename lds2<R>::type;
+  L2 *vs = sh.v[i & 1];
+  const int r = g.r, n = g.n;
+  const bool in = r < n;
+  const cx<R> hri = in ? ld<R>(g.at(r, i)) : mk<R>(0, 0);
+  const R xnorm2 = group_sum<kN>((r >= i + 2 && in) ? abs2(hri) : R(0));
+  const cx<R> alpha = ld<R>(g.at(i + 1, i));
+  cx<R> tau = mk<R>(0, 0), scal = mk<R>(0, 0);
+  R beta = alpha.re;
+  if (!(xnorm2 == R(0) && alpha.im == R(0))) {
+    const R nrm = fsqrt(abs2(alpha) + xnorm2);
+    beta = alpha.re >= R(0) ? -nrm : nrm;
+    const R ib = frcp(beta);
+    tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
+    scal = inv(mk(alpha.re - beta, alpha.im));
+  }
+  cx<R> vr = mk<R>(0, 0);
+  if (r == i + 1) vr = mk<R>(1, 0);
+  else if (r >= i + 2 && in) vr = mul(hri, scal);
+  st<R>(vs + r, vr);
+  if (in && r >= i + 1) st<R>(g.at(r, i), r == i + 1 ? mk<R>(beta, 0) : mk<R>(0, 0));
+  if (r == 0) st<R>(&sh.tau[i & 1], tau);
+  __syncthreads();  // the reflector goes to the Schur-vector engine
+  // H <- H (I - tau v v^H), rows 0..n-1, columns i+1..n-1
+  if (in) {
+    cx<R> wv = mk<R>(0, 0);
+    for (int c = i + 1; c < n; ++c) wv = add(wv, mul(ld<R>(g.at(r, c)), ld<R>(vs + c)));
+    const cx<R> tw = mul(tau, wv);
+    for (int c = i + 1; c < n; ++c) st<R>(g.at(r, c), sub(ld<R>(g.at(r, c)), mulc(tw, ld<R>(vs + c))));
+  }
+  wave_sync();
+  // H <- (I - conj(tau) v v^H) H, rows i+1..n-1, columns i+1..n-1
+  if (in && r >= i + 1) {
+    cx<R> y = mk<R>(0, 0);
+    for (int k = i + 1; k < n; ++k) y = add(y, cmul(ld<R>(vs + k), ld<R>(g.at(k, r))));
+    const cx<R> ty = cmul(tau, y);
+    for (int k = i + 1; k < n; ++k) st<R>(g.at(k, r), sub(ld<R>(g.at(k, r)), mul(ld<R>(vs + k), ty)));
+  }
+  wave_sync();
+}
+
+// Schur-vector engine, reduction step i: Q <- Q (I - tau v v^H) on the row
+// of Q held by this lane.
+template <typename R>
+__device__ void q_step(const LargeShared<R> &sh, cx<R> (&z)[kN], int i) {
+  using L2 = typename lds2<R>::type;
+  __syncthreads();
+  const L2 *vs = sh.v[i & 1];
+  const cx<R> tau = ld<R>(&sh.tau[i & 1]);
+  cx<R> wv = mk<R>(0, 0);
+#pragma unroll
+  for (int c = 0; c < kN; ++c) wv = add(wv, mul(z[c], ld<R>(vs + c)));
+  const cx<R> tw = mul(tau, wv);
+  // re-read v rather than keep 64 loaded values live across the sum
+  int zoff = 0;
+  asm volatile("" : "+s"(zoff));
+  const L2 *vs2 = vs + zoff;
+#pragma unroll
+  for (int c = 0; c < kN; ++c) z[c] = sub(z[c], mulc(tw, ld<R>(vs2 + c)));
+}
+
+// One matrix on the workgroup (w = wavefront index, 0 or 1).  Every
+// workgroup barrier below is reached by both wavefronts the same number of
+// times: once per reduction step, once per QR sweep plus once at the end of
+// the iteration, once when the eigenvector factors are ready.
+//   OUT_V:  E, V = Z X (and T over A with VEIG_WRITE_SCHUR);
+//   OUT_VW: E, V = Z X and W = X^{-1} Z^H ~ V^{-1} in the complex64 planes
+//           rW/iW (the input of the FP64 refinement).
+template <typename R, int OUT>
+__device__ __forceinline__ void solve_one(const FusedArgs &a, LargeShared<R> &sh, int64_t m, int w, int lane) {
+  using G = Group<R, kN, 1>;
   const int n = a.G.n, nmax = a.G.n;
   const int64_t base = a.G.mat_base(m), vbase = a.G.vec_base(m);
-  Gr g;
+  G g;
   g.mi = 0;
   g.r = lane;
   g.n = n;
@@ -265,113 +341,136 @@ __global__ __launch_bounds__(128) void k_zgeev64(FusedArgs a) {
   g.vsh = sh.v[0];
   if (w == 0) {
 #pragma unroll
-    for (int c = 0; c < kN; ++c) st<double>(g.at(lane, c), mk<double>(0, 0));
+    for (int c = 0; c < kN; ++c) st<R>(g.at(lane, c), mk<R>(0, 0));
     wave_sync();
-    load_rows<double, kN, 1>(g, a.G, base, a.rA, a.iA, kN);
+    load_rows<R, kN, 1>(g, a.G, base, a.rA, a.iA, kN);
   }
   __syncthreads();
 
   if (w == 0) {
     // ---------------- Hessenberg engine ----------------
-    for (int i = 0; i + 1 < nmax; ++i) {
-      double2 *vs = sh.v[i & 1];
-      const int r = lane;
-      const bool in = r < n;
-      const cx<double> hri = in ? ld<double>(g.at(r, i)) : mk<double>(0, 0);
-      const double xnorm2 = group_sum<kN>((r >= i + 2 && in) ? abs2(hri) : 0.0);
-      const cx<double> alpha = ld<double>(g.at(i + 1, i));
-      cx<double> tau = mk<double>(0, 0), scal = mk<double>(0, 0);
-      double beta = alpha.re;
-      if (!(xnorm2 == 0.0 && alpha.im == 0.0)) {
-        const double nrm = fsqrt(abs2(alpha) + xnorm2);
-        beta = alpha.re >= 0.0 ? -nrm : nrm;
-        const double ib = frcp(beta);
-        tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
-        scal = inv(mk(alpha.re - beta, alpha.im));
-      }
-      cx<double> vr = mk<double>(0, 0);
-      if (r == i + 1) vr = mk<double>(1, 0);
-      else if (r >= i + 2 && in) vr = mul(hri, scal);
-      st<double>(vs + r, vr);
-      if (in && r >= i + 1) st<double>(g.at(r, i), r == i + 1 ? mk<double>(beta, 0) : mk<double>(0, 0));
-      if (r == 0) st<double>(&sh.tau[i & 1], tau);
-      __syncthreads();  // the reflector goes to the Schur-vector engine
-      // H <- H (I - tau v v^H), rows 0..n-1, columns i+1..n-1
-      if (in) {
-        cx<double> wv = mk<double>(0, 0);
-        for (int c = i + 1; c < n; ++c) wv = add(wv, mul(ld<double>(g.at(r, c)), ld<double>(vs + c)));
-        const cx<double> tw = mul(tau, wv);
-        for (int c = i + 1; c < n; ++c)
-          st<double>(g.at(r, c), sub(ld<double>(g.at(r, c)), mulc(tw, ld<double>(vs + c))));
-      }
-      wave_sync();
-      // H <- (I - conj(tau) v v^H) H, rows i+1..n-1, columns i+1..n-1
-      if (in && r >= i + 1) {
-        cx<double> y = mk<double>(0, 0);
-        for (int k = i + 1; k < n; ++k) y = add(y, cmul(ld<double>(vs + k), ld<double>(g.at(k, r))));
-        const cx<double> ty = cmul(tau, y);
-        for (int k = i + 1; k < n; ++k) st<double>(g.at(k, r), sub(ld<double>(g.at(k, r)), mul(ld<double>(vs + k), ty)));
-      }
-      wave_sync();
-    }
-    const int nfail = lahqr_recorded(g, sh, a.itmax);
-    if (a.flags & VEIG_WRITE_SCHUR) store_rows<double, kN, 1>(g, a.G, base, a.rA, a.iA);
-    const cx<double> lam = trevc_solve_lds<double, kN>(g, nmax);
-    // tile column r becomes -X(:, r): y above the diagonal, -1 on it, 0 below
+    for (int i = 0; i + 1 < nmax; ++i) gehrd_step<R>(g, sh, i);
+    const int nfail = lahqr_recorded<R>(g, sh, a.itmax);
+    if (OUT == OUT_V && (a.flags & VEIG_WRITE_SCHUR)) store_rows<R, kN, 1>(g, a.G, base, a.rA, a.iA);
+    const cx<R> lam = trevc_solve_lds<R, kN>(g, nmax);
     if (lane < n) {
-      st<double>(g.at(lane, lane), mk<double>(-1, 0));
-      for (int i = lane + 1; i < n; ++i) st<double>(g.at(i, lane), mk<double>(0, 0));
-      a.rE[vbase + a.G.ve(lane)] = lam.re;
-      a.iE[vbase + a.G.ve(lane)] = lam.im;
+      if constexpr (OUT == OUT_V) {
+        // tile column r becomes -X(:, r): y above the diagonal, -1 on it, 0 below
+        st<R>(g.at(lane, lane), mk<R>(-1, 0));
+        for (int i = lane + 1; i < n; ++i) st<R>(g.at(i, lane), mk<R>(0, 0));
+      } else {
+        // X above the diagonal (x = -y)
+        for (int i = 0; i < lane; ++i) {
+          const cx<R> y = ld<R>(g.at(i, lane));
+          st<R>(g.at(i, lane), mk(-y.re, -y.im));
+        }
+      }
+      a.rE[vbase + a.G.ve(lane)] = (double)lam.re;
+      a.iE[vbase + a.G.ve(lane)] = (double)lam.im;
+    }
+    if constexpr (OUT == OUT_VW) {
+      wave_sync();
+      inverse_unit_upper<R, kN, 1>(g, nmax);  // Y = X^{-1} below the diagonal
     }
     if (a.info && lane == 0) a.info[m] = nfail;
-    __syncthreads();  // X is ready
+    __syncthreads();  // the eigenvector factors are ready
   } else {
     // ---------------- Schur-vector engine ----------------
-    cx<double> z[kN];
+    cx<R> z[1][kN];
 #pragma unroll
-    for (int c = 0; c < kN; ++c) z[c] = mk<double>(c == lane ? 1.0 : 0.0, 0);
-    for (int i = 0; i + 1 < nmax; ++i) {
-      __syncthreads();
-      const double2 *vs = sh.v[i & 1];
-      const cx<double> tau = ld<double>(&sh.tau[i & 1]);
-      // Q <- Q (I - tau v v^H): row `lane` of Q in registers
-      cx<double> wv = mk<double>(0, 0);
+    for (int c = 0; c < kN; ++c) z[0][c] = mk<R>(c == lane ? R(1) : R(0), 0);
+    for (int i = 0; i + 1 < nmax; ++i) q_step<R>(sh, z[0], i);
+    replay_sweeps<R>(sh, z[0]);
+    __syncthreads();  // the eigenvector factors are ready
+    if constexpr (OUT == OUT_V) {
+      // row `lane` of V = Z X = -Z (-X), column by column, straight to V
+      if (lane < n) {
+        for (int k = 0; k < n; ++k) {
+          cx<R> acc = mk<R>(0, 0);
 #pragma unroll
-      for (int c = 0; c < kN; ++c) wv = add(wv, mul(z[c], ld<double>(vs + c)));
-      const cx<double> tw = mul(tau, wv);
-      // re-read v rather than keep 64 loaded values live across the sum
-      int zoff = 0;
-      asm volatile("" : "+s"(zoff));
-      const double2 *vs2 = vs + zoff;
+          for (int j = 0; j < kN; ++j)
+            if (j < nmax) acc = add(acc, mul(z[0][j], ld<R>(g.at(j, k))));
+          const int64_t idx = base + a.G.el(lane, k);
+          a.rV[idx] = -(double)acc.re;
+          a.iV[idx] = -(double)acc.im;
+        }
+      }
+    } else {
+      // column `lane` of W: W(k, r) = conj(z[k]) + sum_{j>k} Y(k,j) conj(z[j]),
+      // Y(k, j) at tile position (j, k)
 #pragma unroll
-      for (int c = 0; c < kN; ++c) z[c] = sub(z[c], mulc(tw, ld<double>(vs2 + c)));
-    }
-    replay_sweeps(sh, z);
-    __syncthreads();  // X is ready
-    // row `lane` of V = Z X = -Z (-X), column by column, straight to V
-    if (lane < n) {
-      for (int k = 0; k < n; ++k) {
-        cx<double> acc = mk<double>(0, 0);
+      for (int k = 0; k < kN; ++k) {
+        if (k < nmax) {
+          cx<R> acc = conj(z[0][k]);
 #pragma unroll
-        for (int j = 0; j < kN; ++j)
-          if (j < nmax) acc = add(acc, mul(z[j], ld<double>(g.at(j, k))));
-        const int64_t idx = base + a.G.el(lane, k);
-        a.rV[idx] = -acc.re;
-        a.iV[idx] = -acc.im;
+          for (int j = k + 1; j < kN; ++j)
+            if (j < nmax) acc = add(acc, mulc(ld<R>(g.at(j, k)), z[0][j]));
+          if (lane < n) {
+            const int64_t idx = base + a.G.el(k, lane);
+            a.rW[idx] = (float)acc.re;
+            a.iW[idx] = (float)acc.im;
+          }
+        }
+      }
+      backtransform<R, kN, 1>(g, z, nmax);  // row `lane` of V = Z X, in place
+      if (lane < n) {
+#pragma unroll
+        for (int k = 0; k < kN; ++k) {
+          if (k < n) {
+            const int64_t idx = base + a.G.el(lane, k);
+            a.rV[idx] = (double)z[0][k].re;
+            a.iV[idx] = (double)z[0][k].im;
+          }
+        }
       }
     }
+  }
+}
+
+// the batch, one matrix per workgroup
+template <typename R, int OUT>
+__global__ __launch_bounds__(128) void k_zgeev64(FusedArgs a) {
+  __shared__ LargeShared<R> sh;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t m = xcd_block();
+  if (m >= a.G.nmat) return;  // workgroup-uniform
+  solve_one<R, OUT>(a, sh, m, w, lane);
+}
+
+// the matrices of a.list, on a persistent grid
+template <typename R, int OUT>
+__global__ __launch_bounds__(128) void k_zgeev64_list(FusedArgs a) {
+  __shared__ LargeShared<R> sh;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t cnt = *a.list_count;
+  for (int64_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+    solve_one<R, OUT>(a, sh, a.list[i], w, lane);
+    __syncthreads();  // the tile and logs are reused by the next matrix
   }
 }
 
 }  // namespace
 
 namespace veig {
+template <typename R, int OUT>
 int launch_zgeev64(const FusedArgs &a, hipStream_t s) {
   if (a.G.nmat > INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");
-  if (a.list || a.mask) return fail(VEIG_ERR_UNSUPPORTED, "orders above 32: whole batches only");
-  hipLaunchKernelGGL(k_zgeev64, dim3((unsigned)a.G.nmat), dim3(128), 0, s, a);
+  if (a.mask) return fail(VEIG_ERR_UNSUPPORTED, "orders above 32: no masked launches");
+  if (a.list) {
+    // the FP64 solve of the matrices the mixed path hands back
+    if constexpr (std::is_same<R, double>::value && OUT == OUT_V) {
+      const int64_t g = a.G.nmat < kListGrid ? a.G.nmat : kListGrid;
+      hipLaunchKernelGGL((k_zgeev64_list<R, OUT>), dim3((unsigned)g), dim3(128), 0, s, a);
+    } else {
+      return fail(VEIG_ERR_UNSUPPORTED, "orders above 32: list launches are FP64 only");
+    }
+  } else {
+    hipLaunchKernelGGL((k_zgeev64<R, OUT>), dim3((unsigned)a.G.nmat), dim3(128), 0, s, a);
+  }
   VEIG_HIP(hipGetLastError());
   return VEIG_OK;
 }
+template int launch_zgeev64<double, OUT_V>(const FusedArgs &, hipStream_t);
+template int launch_zgeev64<float, OUT_V>(const FusedArgs &, hipStream_t);
+template int launch_zgeev64<float, OUT_VW>(const FusedArgs &, hipStream_t);
 }  // namespace veig

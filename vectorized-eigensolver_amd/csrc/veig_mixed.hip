@@ -6,8 +6,10 @@
 //      triangular X (V = Z X) and Y = X^{-1}; k_refine_zxy then forms
 //      V = Z X and W = Y Z^H ~ V^{-1} on the FP64 matrix cores and refines,
 //      so the VALU kernel does no O(n^3) eigenvector work and keeps its
-//      occupancy.  For 16 < n <= 32 the VALU kernel writes V and W itself.
-//   2. k_refine_mfma<NT> (FP64 matrix cores, one wavefront per matrix):
+//      occupancy.  For 16 < n <= 64 the VALU kernel writes V and W itself
+//      (n > 32: the two-wavefront kernel, complex64).
+//   2. k_refine_mfma<NT> (FP64 matrix cores, one wavefront per matrix; for
+//      n > 32 k_refine_cols, one workgroup of four wavefronts per matrix):
 //      Newton refinement of all eigenpairs at once.  With R = A V - V diag(lam)
 //      and E = W R, the first-order correction of
 //      A (V + dV) = (V + dV)(Lam + dLam) with dV = V C is
@@ -41,6 +43,13 @@ int launch_refine(const RefineArgs &a, hipStream_t s) {
     // the whole batch: coalesced staging through LDS
     const int64_t groups = (a.G.nmat + kStageWaves - 1) / kStageWaves;
     hipLaunchKernelGGL(k_refine_staged, dim3((unsigned)groups), dim3(64 * kStageWaves), 0, s, a);
+    VEIG_HIP(hipGetLastError());
+    return VEIG_OK;
+  }
+  if (a.G.n > 32) {
+    // one matrix per workgroup of four wavefronts
+    const dim3 grid((unsigned)(a.list ? std::min<int64_t>(a.G.nmat, 4096) : a.G.nmat));
+    hipLaunchKernelGGL((k_refine_cols<64>), grid, dim3(64 * kColBlocks), 0, s, a);
     VEIG_HIP(hipGetLastError());
     return VEIG_OK;
   }
@@ -115,7 +124,7 @@ int veig_zgeev_mixed(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const d
                      const double *iA, double *rE, double *iE, double *rV, double *iV,
                      int32_t *info, int32_t *fallback, double *berr, double tol, int32_t itmax,
                      void *workspace, size_t workspace_bytes, void *stream) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV);
   VEIG_REQUIRE(iV); VEIG_REQUIRE(workspace);

@@ -752,4 +752,280 @@ __global__ __launch_bounds__(64 * kStageWaves) __attribute__((amdgpu_waves_per_e
   }
 }
 
+// ---------------------------------------------------------------------------
+// Orders up to 64: one matrix per workgroup of four wavefronts.  Wavefront j
+// owns block column j (16 columns) of V, R = A V - V Lam, E = W R and C in
+// the B layout, one 16 x 16 block per row block I (d4 per lane and block).
+// A and W (row blocks, A layout) are read from global memory where each
+// product needs them; V as the A operand of V C is shared through an LDS
+// tile.  Norms and eigenvalue corrections are combined through LDS at
+// workgroup barriers, in the same order in every wavefront, so the stopping
+// decision is workgroup-uniform.
+// ---------------------------------------------------------------------------
+constexpr int kColBlocks = 4;           // 64 = 4 x 16
+constexpr int kColLd = 16 * kColBlocks + 1;
+
+struct ColShared {
+  double2 t[16 * kColBlocks * kColLd];  // V(row, col) at t[col * kColLd + row]
+  double2 dl[16 * kColBlocks];          // eigenvalue corrections
+  double red[3][kColBlocks];            // per-wavefront partial norms (R, V, A)
+};
+
+struct ColBlock {
+  d4 re[kColBlocks], im[kColBlocks];
+};
+
+// d(I) += M(I, :) x b for I = 0..3: M in the A layout from `ld_a(I, K, s,
+// ar, ai)`, b and d in the B layout (this wavefront's block column).  The
+// 16 k-steps (K, s) are software-pipelined one deep: the operands of step
+// t + 1 are requested before the products of step t, and a scheduling
+// barrier keeps the compiler from hoisting all 64 operand loads at once
+// (which would not fit in registers next to the three block columns).
+template <typename LdA>
+__device__ __forceinline__ void col_gemm(LdA &&ld_a, const ColBlock &b, ColBlock &d) {
+  constexpr int kSteps = 4 * kColBlocks;
+  double ar[2][kColBlocks], ai[2][kColBlocks];
+#pragma unroll
+  for (int I = 0; I < kColBlocks; ++I) ld_a(I, 0, 0, ar[0][I], ai[0][I]);
+#pragma unroll
+  for (int t = 0; t < kSteps; ++t) {
+    const int K = t / 4, s = t % 4, cur = t & 1;
+    if (t + 1 < kSteps) {
+#pragma unroll
+      for (int I = 0; I < kColBlocks; ++I) ld_a(I, (t + 1) / 4, (t + 1) % 4, ar[cur ^ 1][I], ai[cur ^ 1][I]);
+    }
+    const double br = b.re[K][s], bi = b.im[K][s];
+#pragma unroll
+    for (int I = 0; I < kColBlocks; ++I) {
+      d.re[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[cur][I], br, d.re[I], 0, 0, 0);
+      d.im[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[cur][I], bi, d.im[I], 0, 0, 0);
+      d.re[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[cur][I], bi, d.re[I], 0, 0, 0);
+      d.im[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[cur][I], br, d.im[I], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// the sum of the four wavefronts' partials, in a fixed order
+__device__ __forceinline__ double col_total(const double (&red)[kColBlocks]) {
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+template <int NB>
+__device__ __forceinline__ void refine_cols_one(const RefineArgs &a, ColShared &sh, int64_t m) {
+  static_assert(NB == 16 * kColBlocks, "four block columns");
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, g = l >> 4;
+  const int n = a.G.n;
+  const int64_t base = a.G.mat_base(m), vbase = a.G.vec_base(m);
+  const int col = 16 * w + c;  // this lane's column in the B layout
+  // V (B layout), eigenvalue of this lane's column (lc) and of its rows (lr)
+  ColBlock V;
+#pragma unroll
+  for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int row = 16 * I + 4 * s + g;
+      double vr = 0, vi = 0;
+      if (row < n && col < n) {
+        const int64_t idx = base + a.G.el(row, col);
+        vr = a.rV[idx];
+        vi = a.iV[idx];
+      }
+      V.re[I][s] = vr;
+      V.im[I][s] = vi;
+    }
+  cx<double> lc = col < n ? mk<double>(a.rE[vbase + a.G.ve(col)], a.iE[vbase + a.G.ve(col)]) : mk<double>(0, 0);
+  cx<double> lr[kColBlocks][4];
+#pragma unroll
+  for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 16 * I + 4 * s + g;
+      lr[I][s] = k < n ? mk<double>(a.rE[vbase + a.G.ve(k)], a.iE[vbase + a.G.ve(k)]) : mk<double>(0, 0);
+    }
+  // ||A||_F: this wavefront's block column, then the workgroup total
+  double an2 = 0;
+#pragma unroll
+  for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int row = 16 * I + 4 * s + g;
+      if (row < n && col < n) {
+        const int64_t idx = base + a.G.el(row, col);
+        an2 += a.rA[idx] * a.rA[idx] + a.iA[idx] * a.iA[idx];
+      }
+    }
+  an2 = wave_sum(an2);
+  if (l == 0) sh.red[2][w] = an2;
+  __syncthreads();
+  const double nrmA = sqrt(col_total(sh.red[2]));
+  const bool qr_ok = !(a.info32 && a.info32[m] != 0);
+
+  // A operand of a matrix in the batch layout (zero outside n x n); the base
+  // is made opaque so that the loads stay where the products use them
+  auto ld_global = [&](const auto *re, const auto *im) {
+    int64_t b = base, tt = a.G.T;
+    int nn = n;
+    asm volatile("" : "+s"(b), "+s"(tt), "+s"(nn));
+    const int64_t lane_off = b + (int64_t)(c + g * nn) * tt;  // row c, column g
+    const auto *pr = re + lane_off;
+    const auto *pi = im + lane_off;
+    return [=](int I, int K, int s, double &ar, double &ai) {
+      const int row = 16 * I + c, cc = 16 * K + 4 * s + g;
+      const int64_t off = (int64_t)(16 * I + (16 * K + 4 * s) * nn) * tt;  // wavefront-uniform
+      ar = 0;
+      ai = 0;
+      if (row < n && cc < n) {
+        ar = (double)pr[off];
+        ai = (double)pi[off];
+      }
+    };
+  };
+
+  double be = INFINITY, be_prev = INFINITY;
+  for (int it = 0;; ++it) {
+    // ---- R = A V - V diag(lam) and the backward error ----
+    ColBlock Rm;
+#pragma unroll
+    for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double vr = V.re[I][s], vi = V.im[I][s];
+        Rm.re[I][s] = -(vr * lc.re - vi * lc.im);
+        Rm.im[I][s] = -(vr * lc.im + vi * lc.re);
+      }
+    col_gemm(ld_global(a.rA, a.iA), V, Rm);
+    double rn2 = 0, vn2 = 0;
+#pragma unroll
+    for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        rn2 += Rm.re[I][s] * Rm.re[I][s] + Rm.im[I][s] * Rm.im[I][s];
+        vn2 += V.re[I][s] * V.re[I][s] + V.im[I][s] * V.im[I][s];
+      }
+    rn2 = wave_sum(rn2);
+    vn2 = wave_sum(vn2);
+    if (l == 0) {
+      sh.red[0][w] = rn2;
+      sh.red[1][w] = vn2;
+    }
+    __syncthreads();
+    be = sqrt(col_total(sh.red[0])) / (nrmA * sqrt(col_total(sh.red[1])));
+    if (n == 0) be = 0;
+    // workgroup-uniform stop (the same rule as newton_refine); red is next
+    // written after two more barriers of this iteration
+
+    if (!qr_ok || be <= a.tol || it >= a.maxit || !(be < be_prev)) break;
+    if (it > 0) {
+      const double rate = be / be_prev;
+      double reach = be;
+      for (int k = it; k < a.maxit; ++k) reach *= rate;
+      if (reach > a.tol) break;
+    }
+    be_prev = be;
+
+    // ---- E = W R ----
+    ColBlock Em;
+#pragma unroll
+    for (int I = 0; I < kColBlocks; ++I) {
+      Em.re[I] = d4{0, 0, 0, 0};
+      Em.im[I] = d4{0, 0, 0, 0};
+    }
+    col_gemm(ld_global(a.rW, a.iW), Rm, Em);
+    // ---- dlam = diag(E); C = E / (lam_k - lam_i) off the diagonal ----
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (4 * s + g == c) {
+#pragma unroll
+        for (int I = 0; I < kColBlocks; ++I)
+          if (I == w) sh.dl[col] = double2{Em.re[I][s], Em.im[I][s]};
+      }
+#pragma unroll
+    for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int row = 16 * I + 4 * s + g;
+        const cx<double> d = sub(lc, lr[I][s]);
+        const double d2 = abs2(d);
+        const bool zero = row == col || row >= n || col >= n || !(d2 > 0.0);
+        const cx<double> q = zero ? mk<double>(0, 0)
+                                  : scale(mulc(mk<double>(Em.re[I][s], Em.im[I][s]), d), frcp(d2));
+        Em.re[I][s] = q.re;
+        Em.im[I][s] = q.im;
+      }
+    // ---- V <- V + V C, with V shared through the tile ----
+#pragma unroll
+    for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        sh.t[col * kColLd + 16 * I + 4 * s + g] = double2{V.re[I][s], V.im[I][s]};
+    __syncthreads();
+    col_gemm(
+        [&](int I, int K, int s, double &ar, double &ai) {
+          const double2 v = sh.t[(16 * K + 4 * s + g) * kColLd + 16 * I + c];
+          ar = v.x;
+          ai = v.y;
+        },
+        Em, V);
+    {
+      const double2 d = sh.dl[col];
+      lc = add(lc, mk<double>(d.x, d.y));
+    }
+#pragma unroll
+    for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double2 d = sh.dl[16 * I + 4 * s + g];
+        lr[I][s] = add(lr[I][s], mk<double>(d.x, d.y));
+      }
+    __syncthreads();  // the tile and dl are rewritten by the next iteration
+  }
+  const bool ok = (be <= a.tol) && qr_ok;
+  if (a.maxit > 0) {
+#pragma unroll
+    for (int I = 0; I < kColBlocks; ++I)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int row = 16 * I + 4 * s + g;
+        if (row < n && col < n) {
+          const int64_t idx = base + a.G.el(row, col);
+          a.rV[idx] = V.re[I][s];
+          a.iV[idx] = V.im[I][s];
+        }
+      }
+    if (g == 0 && col < n) {
+      a.rE[vbase + a.G.ve(col)] = lc.re;
+      a.iE[vbase + a.G.ve(col)] = lc.im;
+    }
+    if (threadIdx.x == 0) {
+      a.flags[m] = ok ? 0 : 1;
+      if (a.info && ok) a.info[m] = 0;
+    }
+  }
+  if (threadIdx.x == 0 && a.berr) a.berr[m] = be;
+}
+
+// one matrix per workgroup of four wavefronts; the batch, or a.list on a
+// persistent grid
+template <int NB>
+__global__ __launch_bounds__(64 * kColBlocks) void k_refine_cols(RefineArgs a) {
+  __shared__ ColShared sh;
+  // one call site (the body is large): entries i = first, first + step, ...
+  int64_t first, end, step;
+  if (a.list) {
+    first = blockIdx.x;
+    end = *a.list_count;
+    step = gridDim.x;
+  } else {
+    first = xcd_block();
+    end = first < a.G.nmat ? first + 1 : 0;
+    step = 1;
+  }
+  for (int64_t i = first; i < end; i += step) {  // workgroup-uniform
+    const int64_t m = a.list ? a.list[i] : i;
+    if (!a.mask || a.mask[m] != 0) refine_cols_one<NB>(a, sh, m);
+    __syncthreads();
+  }
+}
+
 }  // namespace veig

@@ -21,7 +21,8 @@ HEADER = REPO / "include" / "veig.h"
 
 VEIG_OK = 0
 VEIG_WRITE_SCHUR = 1
-MAX_DIM = 32
+MAX_DIM = 64          # fused entries (veig_zgeev, veig_zgeev_fp32, veig_zgeev_mixed)
+MAX_DIM_STAGES = 32   # stage entries and varsize batches
 
 
 class VeigError(RuntimeError):
