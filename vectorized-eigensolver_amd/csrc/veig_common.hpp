@@ -93,10 +93,10 @@ struct Tiles {
 };
 
 template <typename R, int NB, int P = 1>
-__device__ __forceinline__ Group<R, NB, P> make_group(Tiles<R, NB, P> &sm, int n_of_matrix) {
+__device__ __forceinline__ Group<R, NB, P> make_group(Tiles<R, NB, P> &sm, int n_of_matrix,
+                                                      int lane = threadIdx.x) {
   using G = Group<R, NB, P>;
   G g;
-  const int lane = threadIdx.x;
   g.mi = lane / G::LG;
   g.r = lane % G::LG;
   g.n = n_of_matrix;
@@ -227,6 +227,25 @@ struct FusedArgs {
 };
 
 // What the fused kernel writes: the eigenvectors V; V and an approximate
+// Eigenvector solves of the fused kernels (X above the diagonal of the tile,
+// lam[p] = T(idx(p), idx(p))).  At NB = 32 with one index per lane they run
+// in LDS: the register form would hold NB more complex values per lane next
+// to the Schur vectors.
+template <typename R, int NB, int P>
+__device__ __forceinline__ void solve_eigvecs(const Group<R, NB, P> &g, int nmax, cx<R> (&lam)[P]) {
+  if constexpr (P == 1 && NB == 32) {
+    lam[0] = trevc_solve_lds<R, NB>(g, nmax);  // leaves y = -x
+    if (g.r < g.n)
+      for (int i = 0; i < g.r; ++i) {
+        const cx<R> y = ld<R>(g.at(i, g.r));
+        st<R>(g.at(i, g.r), mk(-y.re, -y.im));
+      }
+    wave_sync();
+  } else {
+    trevc_solve<R, NB, P>(g, nmax, lam);
+  }
+}
+
 // inverse W = V^{-1} (complex64); or, for refinement on the matrix cores, the
 // Schur vectors Z and the eigenvector factor X of T (V = Z X) with
 // Y = X^{-1} (complex64), instead of V.
@@ -234,12 +253,15 @@ enum { OUT_V = 0, OUT_VW = 1, OUT_ZXY = 2 };
 
 // one wavefront of the fused kernel: MPW matrices, from the batch (wave
 // index `wave`) or from a.list (list entries wave*MPW ..)
+// (lane: threadIdx.x; the list kernel passes it through an opaque copy so
+// that the LDS addresses derived from it are recomputed for every matrix
+// instead of being hoisted out of its loop and held in registers)
 template <typename R, int NB, int P, int OUT>
 __device__ __forceinline__ void zgeev_wave(const FusedArgs &a, Tiles<R, NB, P> &sm, int64_t wave,
-                                           int64_t list_count) {
+                                           int64_t list_count, int lane) {
   using G = Group<R, NB, P>;
   constexpr int MPW = G::MPW;
-  const int64_t slot = wave * MPW + threadIdx.x / G::LG;
+  const int64_t slot = wave * MPW + lane / G::LG;
   int64_t m;
   bool valid;
   if (a.list) {
@@ -252,7 +274,7 @@ __device__ __forceinline__ void zgeev_wave(const FusedArgs &a, Tiles<R, NB, P> &
   }
   if (!__any(valid)) return;  // wave-uniform early exit (masked launches)
   const int n = valid ? a.G.n : 0;
-  G g = make_group<R, NB, P>(sm, n);
+  G g = make_group<R, NB, P>(sm, n, lane);
   const int64_t base = valid ? a.G.mat_base(m) : 0;
   const int64_t vbase = valid ? a.G.vec_base(m) : 0;
   wave_sync();
@@ -283,7 +305,7 @@ __device__ __forceinline__ void zgeev_wave(const FusedArgs &a, Tiles<R, NB, P> &
     return;
   }
   cx<R> lam[P];
-  trevc_solve<R, NB, P>(g, nmax, lam);
+  solve_eigvecs<R, NB, P>(g, nmax, lam);
   if constexpr (OUT == OUT_ZXY) {
     // Z first, so that its registers are free for the triangular inverse
 #pragma unroll
@@ -347,7 +369,7 @@ template <typename R, int NB, int P, int OUT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(zgeev_waves_per_eu<R, NB, P, OUT>())))
 void k_zgeev(FusedArgs a) {
   __shared__ Tiles<R, NB, P> sm;
-  zgeev_wave<R, NB, P, OUT>(a, sm, xcd_block(), 0);
+  zgeev_wave<R, NB, P, OUT>(a, sm, xcd_block(), 0, threadIdx.x);
 }
 
 // the matrices of a.list, on a persistent grid
@@ -358,7 +380,9 @@ __global__ __launch_bounds__(64) void k_zgeev_list(FusedArgs a) {
   const int64_t cnt = *a.list_count;
   const int64_t nw = (cnt + MPW - 1) / MPW;
   for (int64_t w = blockIdx.x; w < nw; w += gridDim.x) {
-    zgeev_wave<R, NB, P, OUT>(a, sm, w, cnt);
+    int lane = threadIdx.x;
+    asm volatile("" : "+v"(lane));
+    zgeev_wave<R, NB, P, OUT>(a, sm, w, cnt, lane);
     wave_sync();
   }
 }

@@ -444,7 +444,11 @@ __global__ __launch_bounds__(128) void k_zgeev64_list(FusedArgs a) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t cnt = *a.list_count;
   for (int64_t i = blockIdx.x; i < cnt; i += gridDim.x) {
-    solve_one<R, OUT>(a, sh, a.list[i], w, lane);
+    // an opaque copy of the lane: addresses derived from it are recomputed
+    // for every matrix instead of being hoisted out of the loop
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    solve_one<R, OUT>(a, sh, a.list[i], w, ln);
     __syncthreads();  // the tile and logs are reused by the next matrix
   }
 }

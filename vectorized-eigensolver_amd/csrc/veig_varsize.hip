@@ -123,7 +123,11 @@ __global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
   const int64_t start = a.bins->start[bin], count = a.bins->count[bin];
   const int64_t nwaves = (count + MPW - 1) / MPW;
   for (int64_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
-    const int64_t idx = w * MPW + threadIdx.x / Gr::LG;
+    // an opaque copy of the lane: addresses derived from it are recomputed
+    // for every wavefront of matrices instead of being hoisted out of the loop
+    int lane = threadIdx.x;
+    asm volatile("" : "+v"(lane));
+    const int64_t idx = w * MPW + lane / Gr::LG;
     const bool valid = idx < count;
     const int64_t m = valid ? a.perm[start + idx] : 0;
     const int n = valid ? a.dims[m] : 0;
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
       const int nq = __builtin_amdgcn_readlane(n, q * Gr::LG);
       nmax = nq > nmax ? nq : nmax;
     }
-    Gr g = make_group<R, NB, P>(sm, n);
+    Gr g = make_group<R, NB, P>(sm, n, lane);
     Geom G;
     G.nmat = 0;
     G.T = 1;
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
     gehrd_q<R, NB, P>(g, z, nmax, true, false, nullptr);
     const int nfail = lahqr<R, NB, P>(g, z, a.itmax, false);
     cx<R> lam[P];
-    trevc_solve<R, NB, P>(g, nmax, lam);
+    solve_eigvecs<R, NB, P>(g, nmax, lam);
     backtransform<R, NB, P>(g, z, nmax);
     store_row_regs<R, NB, P>(g, G, base, a.rV, a.iV, z);
     store_eigenvalues<R, NB, P>(g, G, vbase, a.rE, a.iE, lam);
