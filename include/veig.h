@@ -181,6 +181,33 @@ int veig_backsubstitution(int64_t dim, int64_t block_cnt, int64_t thread_cnt, co
                           const double *iT, double *rb, double *ib, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Eigenvalues only, and polynomial roots, device pointers.
+ * ------------------------------------------------------------------------- */
+/* zlahqr_ (Toolbox.hpp:1663-1890, the eigenvalues-only QR) on every matrix
+ * of the batch: H upper Hessenberg with a real subdiagonal (entries below the
+ * subdiagonal are ignored), updated in place inside the active blocks only
+ * (as the reference; T is not formed), E = the eigenvalues in deflation
+ * order.  info as veig_zgeev. */
+int veig_zlahqr_eig(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rE, double *iE,
+                    double *rH, double *iH, int32_t *info, int32_t itmax, void *stream);
+/* Eigenvalues of general matrices: the Hessenberg reduction without Q, then
+ * the iteration of veig_zlahqr_eig (A read only).  No single counterpart in
+ * the reference (zgehrd followed by its zlahqr_); E is, bit for bit, the E of
+ * veig_zgeev on the same input. */
+int veig_eigvals(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA, const double *iA,
+                 double *rE, double *iE, int32_t *info, int32_t itmax, void *stream);
+/* The polynomial-roots kernel zlahqr (Toolbox.hpp:1894-1957): for each
+ * polynomial c_0 + c_1 x + ... + c_dim x^dim (coefficients rC/iC in the
+ * vector layout with dim+1 entries per polynomial) the companion matrix of
+ * create_companion_matrices (Toolbox.hpp:605-643) is iterated to Schur form;
+ * E = the roots.  With rV, iV, rZ, iZ (all four) V receives the Schur form T
+ * and Z the Schur vectors, as in the reference; with all four NULL only the
+ * roots are computed (the eigenvalues-only iteration). */
+int veig_zlahqr_roots(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rC,
+                      const double *iC, double *rE, double *iE, double *rV, double *iV, double *rZ,
+                      double *iZ, int32_t *info, int32_t itmax, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Layout helpers on the device (utils.cpp:85-157).  `in` and `out` must not
  * overlap.  AoS = one column-major n x n matrix after the other.
  * ------------------------------------------------------------------------- */

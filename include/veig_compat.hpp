@@ -60,6 +60,12 @@ void hessenberg(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, 
                 double *iQ);
 void eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rE,
                  double *iE, double *rV, double *iV);
+// loadPackages (utils.cpp:17-50): `packages` files testcases/f_coeffs_<k>.csv
+// (k = 1..packages, relative to the working directory), each with `matrices`
+// lines of 2*n_coeffs comma-separated numbers (re, im of c_0 .. c_{n-1});
+// C[coeff + line*n_coeffs + package*matrices*n_coeffs].  Throws veig::Error
+// if a file cannot be opened (the reference reads nothing and goes on).
+void loadPackages(int packages, int matrices, int n_coeffs, ComplexDouble *C);
 
 // batch entry points (Toolbox-def.hpp:8-22).  The work arrays are accepted
 // and unused: the kernels keep their work on chip.
@@ -70,6 +76,11 @@ void test_ztrevc(double *rV, double *iV, double *rZ, double *iZ, double *rwork, 
 void zlahqr_new(double *rE, double *iE, double *rV, double *iV, double *rZ, double *iZ, int dim,
                 int dim2, int block_cnt, int thread_cnt);
 void test_ident(double *rZ, double *iZ, int dim, int block_cnt, int thread_cnt);
+// the polynomial-roots kernel zlahqr (Toolbox.hpp:1894): dim1 = dim + 1
+// coefficients per polynomial, dim2 = dim * dim; E = roots, V = Schur form of
+// the companion matrix, Z = Schur vectors
+void zlahqr(double *rC, double *iC, double *rE, double *iE, double *rV, double *iV, double *rZ,
+            double *iZ, int dim, int dim1, int dim2, int block_cnt, int thread_cnt);
 void test_diag(double *rM, double *iM, double real, double imag, int dim, int block_cnt,
                int thread_cnt);
 void test_zgehrd(double *rA, double *iA, double *rtau, double *itau, double *rwork, double *iwork,

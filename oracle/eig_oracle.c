@@ -371,8 +371,9 @@ static void qr_sweep(mat_t H, mat_t Z, int I, int M, int I1, int I2, double v0r,
       H.re[IX(H, j, K)] = h1r; H.im[IX(H, j, K)] = h1i;
       H.re[IX(H, j, K + 1)] = h2r; H.im[IX(H, j, K + 1)] = h2i;
     }
-    /* Schur vectors, columns K, K+1, all rows (loop 100) */
-    for (int j = 1; j <= n; ++j) {
+    /* Schur vectors, columns K, K+1, all rows (loop 100; absent in the
+     * eigenvalues-only single_shift_qr, Toolbox.hpp:372-522) */
+    for (int j = 1; Z.re && j <= n; ++j) {
       double z1r = Z.re[IX(Z, j, K)], z1i = Z.im[IX(Z, j, K)];
       double z2r = Z.re[IX(Z, j, K + 1)], z2i = Z.im[IX(Z, j, K + 1)];
       double zr = z1r, zi = z1i;
@@ -398,11 +399,14 @@ static void qr_sweep(mat_t H, mat_t Z, int I, int M, int I1, int I2, double v0r,
     H.im[IX(H, I, I - 1)] = 0.0;
     for (int j = I + 1; j <= I2; ++j) c_mul(&H.re[IX(H, I, j)], &H.im[IX(H, I, j)], zr, zi);
     for (int j = I1; j <= I - 1; ++j) c_mulc(&H.re[IX(H, j, I)], &H.im[IX(H, j, I)], zr, zi);
-    for (int j = 1; j <= n; ++j) c_mulc(&Z.re[IX(Z, j, I)], &Z.im[IX(Z, j, I)], zr, zi);
+    for (int j = 1; Z.re && j <= n; ++j) c_mulc(&Z.re[IX(Z, j, I)], &Z.im[IX(Z, j, I)], zr, zi);
   }
 }
 
-/* ---- zlahqr_schur(wantT=1, wantZ=1), Toolbox.hpp:690-1031 ---- */
+/* ---- zlahqr_schur(wantT=1, wantZ=1), Toolbox.hpp:690-1031; with
+ * ORACLE_EIG_ONLY the eigenvalues-only zlahqr_ (Toolbox.hpp:1663-1890): no
+ * loop 20, no Z (Z.re == NULL), and each sweep updates H only inside the
+ * active block (I1 = L, I2 = I, 1761-1762; single_shift_qr 372-522). ---- */
 static void zlahqr_one(mat_t H, mat_t Z, double *rE, double *iE, int flags, int32_t *info,
                        int32_t *sweeps, int32_t *steps) {
   const int n = H.n;
@@ -410,10 +414,11 @@ static void zlahqr_one(mat_t H, mat_t Z, double *rE, double *iE, int flags, int3
   const int ILO = 1, IHI = n;
   const double safemin = DBL_MIN, ulp = DBL_EPSILON;
   const double smlnum = safemin * (((double)(IHI - ILO + 1)) / ulp);
+  const int eig_only = (flags & ORACLE_EIG_ONLY) != 0;
   int nfail = 0, nsweep = 0, nstep = 0;
 
   /* loop 20: make the subdiagonal real (735-793) */
-  for (int i = ILO + 1; i <= IHI; ++i) {
+  for (int i = ILO + 1; !eig_only && i <= IHI; ++i) {
     const double v0i = H.im[IX(H, i, i - 1)];
     if (v0i != 0.0) {
       const double v0r = H.re[IX(H, i, i - 1)];
@@ -433,7 +438,7 @@ static void zlahqr_one(mat_t H, mat_t Z, double *rE, double *iE, int flags, int3
     }
   }
 
-  const int I1 = 1, I2 = n;
+  int I1 = 1, I2 = n;
   for (int I = IHI; I >= ILO; --I) {
     int L = ILO, K = I, converged = 0;
     for (int its = 0; its <= 30; ++its) {
@@ -469,6 +474,10 @@ static void zlahqr_one(mat_t H, mat_t Z, double *rE, double *iE, int flags, int3
       if (L >= I) {
         converged = 1;
         break;
+      }
+      if (eig_only) {
+        I1 = L;
+        I2 = I;
       }
       /* shift (896-990) */
       double tr, ti;
@@ -721,6 +730,70 @@ void oracle_call_zlahqr_new(int dim, int block_cnt, int thread_cnt, double *rE, 
       }
     }
   oracle_zlahqr(dim, block_cnt, thread_cnt, rE, iE, rM, iM, rS, iS, flags, NULL);
+}
+
+/* zlahqr_ (eigenvalues only), Toolbox.hpp:1663-1890, over the batch: H is
+ * upper Hessenberg with a real subdiagonal and is updated inside the active
+ * blocks only; E receives its diagonal. */
+void oracle_zlahqr_eig(int dim, int block_cnt, int thread_cnt, double *rE, double *iE, double *rH,
+                       double *iH, int flags, oracle_stats *stats) {
+  const mat_t none = {NULL, NULL, dim, (size_t)thread_cnt};
+#pragma omp parallel for schedule(static, 8) num_threads(OMP_THREADS())
+  for (int b = 0; b < block_cnt; ++b)
+    for (int t = 0; t < thread_cnt; ++t) {
+      const size_t v = vslot(dim, thread_cnt, b, t, 1);
+      const size_t m = (size_t)b * thread_cnt + t;
+      zlahqr_one(slot(rH, iH, dim, thread_cnt, b, t), none, rE + v, iE + v, flags | ORACLE_EIG_ONLY,
+                 stats && stats->info ? stats->info + m : NULL,
+                 stats && stats->sweeps ? stats->sweeps + m : NULL,
+                 stats && stats->steps ? stats->steps + m : NULL);
+    }
+}
+
+/* create_companion_matrices, Toolbox.hpp:605-643: the coefficients
+ * c_0..c_dim of each polynomial (vector slot of length dim+1) give the
+ * companion matrix with ones on the subdiagonal and last column
+ * -c_i / c_dim (Smith division, ComplexDouble.hpp:319-353). */
+void oracle_companion(int dim, int block_cnt, int thread_cnt, const double *rC, const double *iC,
+                      double *rM, double *iM) {
+  for (int b = 0; b < block_cnt; ++b)
+    for (int t = 0; t < thread_cnt; ++t) {
+      const size_t v = vslot(dim + 1, thread_cnt, b, t, 1);
+      const size_t T = (size_t)thread_cnt;
+      mat_t M = slot(rM, iM, dim, thread_cnt, b, t);
+      for (int c = 1; c <= dim; ++c)
+        for (int r = 1; r <= dim; ++r) {
+          M.re[IX(M, r, c)] = 0.0;
+          M.im[IX(M, r, c)] = 0.0;
+        }
+      const double cr = rC[v + (size_t)dim * T], ci = iC[v + (size_t)dim * T];
+      for (int i = 0; i < dim; ++i) {
+        double a = rC[v + (size_t)i * T], bb = iC[v + (size_t)i * T];
+        c_div(&a, &bb, cr, ci);
+        M.re[IX(M, i + 1, dim)] = -a;
+        M.im[IX(M, i + 1, dim)] = -bb;
+      }
+      for (int i = 1; i < dim; ++i) M.re[IX(M, i + 1, i)] = 1.0;
+    }
+}
+
+/* zlahqr kernel (polynomial roots), Toolbox.hpp:1894-1957: companion matrix
+ * into V, Z = I, zlahqr_schur(wantT=1, wantZ=1); E receives the roots, V the
+ * Schur form T and Z the Schur vectors. */
+void oracle_zlahqr_roots(int dim, int block_cnt, int thread_cnt, const double *rC, const double *iC,
+                         double *rE, double *iE, double *rV, double *iV, double *rZ, double *iZ,
+                         int flags, oracle_stats *stats) {
+  oracle_companion(dim, block_cnt, thread_cnt, rC, iC, rV, iV);
+  for (int b = 0; b < block_cnt; ++b)
+    for (int t = 0; t < thread_cnt; ++t) {
+      mat_t Z = slot(rZ, iZ, dim, thread_cnt, b, t);
+      for (int c = 1; c <= dim; ++c)
+        for (int r = 1; r <= dim; ++r) {
+          Z.re[IX(Z, r, c)] = r == c ? 1.0 : 0.0;
+          Z.im[IX(Z, r, c)] = 0.0;
+        }
+    }
+  oracle_zlahqr(dim, block_cnt, thread_cnt, rE, iE, rV, iV, rZ, iZ, flags, stats);
 }
 
 void oracle_ztrevc(int dim, int block_cnt, int thread_cnt, double *rT, double *iT, double *rR,

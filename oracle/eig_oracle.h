@@ -35,7 +35,11 @@ enum {
   ORACLE_NO_T_RESCALE = 8,
   /* Hermitian zgemv reading the imaginary part of y (LAPACK) instead of the
    * reference's iv = ry[...] (Toolbox.hpp:1125). */
-  ORACLE_ZGEMV_FIX = 16
+  ORACLE_ZGEMV_FIX = 16,
+  /* QR stage as the eigenvalues-only zlahqr_ (Toolbox.hpp:1663-1890): no Z,
+   * no initial real-subdiagonal pass, updates inside the active block only.
+   * Set by oracle_zlahqr_eig. */
+  ORACLE_EIG_ONLY = 32
 };
 
 /* Per-matrix statistics from the QR stage (any pointer may be NULL).
@@ -58,6 +62,19 @@ void oracle_zlahqr(int dim, int block_cnt, int thread_cnt, double *rE, double *i
                    double *iH, double *rZ, double *iZ, int flags, oracle_stats *stats);
 void oracle_call_zlahqr_new(int dim, int block_cnt, int thread_cnt, double *rE, double *iE,
                             double *rM, double *iM, double *rS, double *iS, int flags);
+/* Eigenvalues only (zlahqr_, Toolbox.hpp:1663): H upper Hessenberg with a
+ * real subdiagonal, updated in place inside the active blocks. */
+void oracle_zlahqr_eig(int dim, int block_cnt, int thread_cnt, double *rE, double *iE, double *rH,
+                       double *iH, int flags, oracle_stats *stats);
+/* Companion matrices of polynomials c_0 + c_1 x + ... + c_dim x^dim, the
+ * coefficients a vector slot of length dim+1 (Toolbox.hpp:605-643). */
+void oracle_companion(int dim, int block_cnt, int thread_cnt, const double *rC, const double *iC,
+                      double *rM, double *iM);
+/* Polynomial roots as the reference's zlahqr kernel (Toolbox.hpp:1894):
+ * E = roots, V = Schur form of the companion matrix, Z = Schur vectors. */
+void oracle_zlahqr_roots(int dim, int block_cnt, int thread_cnt, const double *rC, const double *iC,
+                         double *rE, double *iE, double *rV, double *iV, double *rZ, double *iZ,
+                         int flags, oracle_stats *stats);
 void oracle_ztrevc(int dim, int block_cnt, int thread_cnt, double *rT, double *iT, double *rR,
                    double *iR, int flags);
 void oracle_eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rE,

@@ -15,6 +15,8 @@
 //   call_ztrevc                                  utils.cpp:279
 //   hessenberg                                   utils.cpp:417
 //   call_zgemv / call_backsubstitution           utils.cpp:218 / 269
+//   zlahqr_ (eigenvalues only)                   Toolbox.hpp:1663
+//   zlahqr (polynomial roots kernel)             Toolbox.hpp:1894
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -29,6 +31,12 @@ struct ComplexDouble;
 void test_diag(double *rM, double *iM, double real, double imag, int dim, int block_cnt, int thread_cnt);
 void test_backsubstitution(double *rT, double *iT, double *rx, double *ix, int dim, int block_cnt,
                            int thread_cnt);
+// Defined in Toolbox.hpp (1663, 1894) and not declared in its headers: the
+// eigenvalues-only QR of one matrix slot and the polynomial-roots kernel
+// (a host loop over the batch in this CPU build).
+void zlahqr_(double *rE, double *iE, double *rV, double *iV, int dim, int block_cnt, int thread_cnt);
+void zlahqr(double *rC, double *iC, double *rE, double *iE, double *rV, double *iV, double *rZ, double *iZ,
+            int dim, int dim1, int dim2, int block_cnt, int thread_cnt);
 // Declared only in the Cython binding (python_interface.pyx:24-25).
 void call_backsubstitution(int dim, int block_cnt, int thread_cnt, double *rT, double *iT, double *rb,
                            double *ib);
@@ -130,6 +138,21 @@ void ref_zgemv(int hermitian, int dim, int block_cnt, int thread_cnt, double *rM
 void ref_backsubstitution(int dim, int block_cnt, int thread_cnt, double *rT, double *iT,
                           double *rb, double *ib) {
   call_backsubstitution(dim, block_cnt, thread_cnt, rT, iT, rb, ib);
+}
+// zlahqr_ on every matrix slot of the batch (H Hessenberg, updated in place).
+void ref_zlahqr_eig(int dim, int block_cnt, int thread_cnt, double *rE, double *iE, double *rH,
+                    double *iH) {
+  const size_t T = thread_cnt, n2 = (size_t)dim * dim;
+  for (int b = 0; b < block_cnt; ++b)
+    for (int t = 0; t < thread_cnt; ++t) {
+      const size_t m = (size_t)b * T * n2 + t, v = (size_t)b * T * dim + t;
+      zlahqr_(rE + v, iE + v, rH + m, iH + m, dim, block_cnt, thread_cnt);
+    }
+}
+// The roots kernel as shipped: coefficients c_0..c_dim per polynomial.
+void ref_zlahqr_roots(int dim, int block_cnt, int thread_cnt, double *rC, double *iC, double *rE,
+                      double *iE, double *rV, double *iV, double *rZ, double *iZ) {
+  zlahqr(rC, iC, rE, iE, rV, iV, rZ, iZ, dim, dim + 1, dim * dim, block_cnt, thread_cnt);
 }
 
 }  // extern "C"

@@ -26,6 +26,7 @@ LAPACK_SIGN = 2
 LAPACK_TST = 4
 NO_T_RESCALE = 8
 ZGEMV_FIX = 16
+EIG_ONLY = 32
 
 _P = ctypes.POINTER(ctypes.c_double)
 _I32P = ctypes.POINTER(ctypes.c_int32)
@@ -95,6 +96,9 @@ def load_oracle():
     lib.oracle_zgemv.argtypes = [i, i, i, i] + [_P] * 6 + [d, d, i]
     lib.oracle_backsubstitution.argtypes = [i, i, i] + [_P] * 4
     lib.oracle_set_threads.argtypes = [i]
+    lib.oracle_zlahqr_eig.argtypes = [i, i, i] + [_P] * 4 + [i, ctypes.POINTER(_Stats)]
+    lib.oracle_companion.argtypes = [i, i, i] + [_P] * 4
+    lib.oracle_zlahqr_roots.argtypes = [i, i, i] + [_P] * 8 + [i, ctypes.POINTER(_Stats)]
     return lib
 
 
@@ -111,6 +115,9 @@ def load_ref(exact: bool = True):
     lib.ref_zlahqr_new.argtypes = [i, i, i] + [_P] * 6
     lib.ref_call_zlahqr_new.argtypes = [i, i, i] + [_P] * 6
     lib.ref_zgemv.argtypes = [i, i, i, i] + [_P] * 6 + [d, d]
+    if hasattr(lib, "ref_zlahqr_eig"):
+        lib.ref_zlahqr_eig.argtypes = [i, i, i] + [_P] * 4
+        lib.ref_zlahqr_roots.argtypes = [i, i, i] + [_P] * 8
     return lib
 
 
@@ -144,6 +151,53 @@ def ref_eig(A: np.ndarray, thread_cnt: int = 1, clean: bool = True, lib=None):
     fn(n, N // thread_cnt, thread_cnt, dptr(rA), dptr(iA), dptr(rE), dptr(iE), dptr(rV), dptr(iV))
     return (vec_from_soa(rE, iE, n, thread_cnt), from_soa(rV, iV, n, thread_cnt),
             from_soa(rA, iA, n, thread_cnt))
+
+
+def hessenberg_input(N: int, n: int, seed: int, thread_cnt: int = 1, lib=None) -> np.ndarray:
+    """Upper Hessenberg matrices with a real subdiagonal: the oracle's
+    reduction of Gaussian matrices, entries below the subdiagonal cleared."""
+    lib = lib or load_oracle()
+    A = gaussian(N, n, seed)
+    rA, iA = to_soa(A, thread_cnt)
+    rQ = np.zeros_like(rA); iQ = np.zeros_like(iA)
+    lib.oracle_hessenberg(n, N // thread_cnt, thread_cnt, dptr(rA), dptr(iA), dptr(rQ), dptr(iQ), 0)
+    return np.triu(from_soa(rA, iA, n, thread_cnt), -1)
+
+
+def eig_only(H: np.ndarray, thread_cnt: int = 1, lib=None, ref: bool = False):
+    """Eigenvalues of Hessenberg H by the oracle's (or the reference's)
+    eigenvalues-only QR.  Returns (E, H after the iteration, info)."""
+    N, n, _ = H.shape
+    rH, iH = to_soa(H, thread_cnt)
+    rE = np.zeros(N * n); iE = np.zeros(N * n)
+    info = np.zeros(N, np.int32)
+    if ref:
+        lib.ref_zlahqr_eig(n, N // thread_cnt, thread_cnt, dptr(rE), dptr(iE), dptr(rH), dptr(iH))
+    else:
+        lib = lib or load_oracle()
+        S = _Stats(info.ctypes.data_as(_I32P), None, None)
+        lib.oracle_zlahqr_eig(n, N // thread_cnt, thread_cnt, dptr(rE), dptr(iE), dptr(rH), dptr(iH), 0,
+                              ctypes.byref(S))
+    return vec_from_soa(rE, iE, n, thread_cnt), from_soa(rH, iH, n, thread_cnt), info
+
+
+def roots_run(Cf: np.ndarray, thread_cnt: int = 1, lib=None, ref: bool = False):
+    """Polynomial roots by the oracle's (or the reference's) roots kernel.
+    Cf: (N, n+1) complex coefficients c_0..c_n.  Returns (E, T, Z)."""
+    N, n1 = Cf.shape
+    n = n1 - 1
+    rC, iC = vec_to_soa(Cf, thread_cnt)
+    rE = np.zeros(N * n); iE = np.zeros(N * n)
+    rV = np.zeros(N * n * n); iV = np.zeros(N * n * n)
+    rZ = np.zeros(N * n * n); iZ = np.zeros(N * n * n)
+    args = [dptr(rC), dptr(iC), dptr(rE), dptr(iE), dptr(rV), dptr(iV), dptr(rZ), dptr(iZ)]
+    if ref:
+        lib.ref_zlahqr_roots(n, N // thread_cnt, thread_cnt, *args)
+    else:
+        lib = lib or load_oracle()
+        lib.oracle_zlahqr_roots(n, N // thread_cnt, thread_cnt, *args, 0, None)
+    return (vec_from_soa(rE, iE, n, thread_cnt), from_soa(rV, iV, n, thread_cnt),
+            from_soa(rZ, iZ, n, thread_cnt))
 
 
 # --------------------------------------------------------------------------

@@ -48,6 +48,17 @@ def test_argument_validation_without_device():
         veig.check(L.veig_zgehrd(-3, 1, 1, None, None, None, None, None), "veig_zgehrd")
 
 
+def test_roots_and_eigvals_validation_without_device():
+    L = veig.lib()
+    d = [ctypes.c_void_p(8)] * 4  # never dereferenced: validation fails first
+    # Schur outputs: all four or none
+    st = L.veig_zlahqr_roots(6, 1, 1, *d, ctypes.c_void_p(8), None, None, None, None, 30, None)
+    assert st == 1 and b"all four" in L.veig_last_error()
+    assert L.veig_zlahqr_roots(65, 1, 1, *d, None, None, None, None, None, 30, None) == 2
+    assert L.veig_eigvals(8, 1, 1, None, None, None, None, None, 30, None) == 1
+    assert L.veig_zlahqr_eig(0, 3, 2, None, None, None, None, None, 30, None) == 0
+
+
 def test_mixed_workspace_size():
     L = veig.lib()
     n, bc, tc = 16, 16384, 64

@@ -35,6 +35,8 @@ MANGLED = {
     "test_ident": "_Z10test_identPdS_iii",
     "test_diag": "_Z9test_diagPdS_ddiii",
     "test_backsubstitution": "_Z21test_backsubstitutionPdS_S_S_iii",
+    "zlahqr": "_Z6zlahqrPdS_S_S_S_S_S_S_iiiii",
+    "loadPackages": "_Z12loadPackagesiiiP13ComplexDouble",
 }
 
 
@@ -92,3 +94,26 @@ def test_diag_and_ident():
     g.argtypes = [ctypes.POINTER(ctypes.c_double)] * 2 + [ctypes.c_int] * 3
     g(H.dptr(r), H.dptr(i), dim, bc, tc)
     assert np.array_equal(H.from_soa(r, i, dim, tc), np.broadcast_to(np.eye(dim), M.shape))
+
+
+def test_load_packages_reads_reference_csv(tmp_path, monkeypatch):
+    # utils.cpp:17-50: testcases/f_coeffs_<k>.csv, one polynomial per line,
+    # re,im pairs of c_0..c_{n-1}; C[coeff + line*n + package*matrices*n]
+    rng = np.random.default_rng(3)
+    packages, matrices, n = 2, 3, 4
+    want = np.zeros(packages * matrices * n, complex)
+    (tmp_path / "testcases").mkdir()
+    for k in range(packages):
+        lines = []
+        for row in range(matrices):
+            c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            want[k * matrices * n + row * n: k * matrices * n + (row + 1) * n] = c
+            lines.append(",".join(f"{v:.17g}" for z in c for v in (z.real, z.imag)))
+        (tmp_path / "testcases" / f"f_coeffs_{k + 1}.csv").write_text("\n".join(lines) + "\n")
+    monkeypatch.chdir(tmp_path)
+    out = np.zeros(packages * matrices * n, complex)
+    f = sym("loadPackages")
+    f.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p]
+    f.restype = None
+    f(packages, matrices, n, out.ctypes.data)
+    assert np.array_equal(out, want)

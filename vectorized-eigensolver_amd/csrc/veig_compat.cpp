@@ -5,7 +5,10 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <vector>
 
 namespace {
@@ -117,6 +120,25 @@ void hessenberg(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, 
   check(veig_host_hessenberg(dim, block_cnt, thread_cnt, rA, iA, rQ, iQ), "hessenberg");
 }
 
+void loadPackages(int packages, int matrices, int n_coeffs, ComplexDouble *C) {
+  for (int package = 0; package < packages; ++package) {
+    const std::string path = "testcases/f_coeffs_" + std::to_string(package + 1) + ".csv";
+    std::ifstream in(path);
+    if (!in) throw veig::Error(VEIG_ERR_INVALID, "loadPackages: cannot open " + path);
+    std::string line;
+    for (int row = 0; row < matrices; ++row) {
+      std::getline(in, line);
+      std::stringstream ls(line);
+      std::vector<double> v(2 * (size_t)n_coeffs, 0.0);
+      std::string cell;
+      // as the reference: missing or malformed cells read as 0 (atof)
+      for (size_t k = 0; k < v.size() && std::getline(ls, cell, ','); ++k) v[k] = std::atof(cell.c_str());
+      for (int c = 0; c < n_coeffs; ++c)
+        C[c + (size_t)row * n_coeffs + (size_t)package * matrices * n_coeffs] = ComplexDouble(v[2 * c], v[2 * c + 1]);
+    }
+  }
+}
+
 void eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rE,
                  double *iE, double *rV, double *iV) {
   check(veig_host_eigensolver(dim, block_cnt, thread_cnt, rA, iA, rE, iE, rV, iV), "eigensolver");
@@ -149,6 +171,27 @@ void zlahqr_new(double *rE, double *iE, double *rV, double *iV, double *rZ, doub
         "zlahqr_new");
   sync();
   h1.back(rV); h2.back(iV); z1.back(rZ); z2.back(iZ); e1.back(rE); e2.back(iE);
+}
+
+void zlahqr(double *rC, double *iC, double *rE, double *iE, double *rV, double *iV, double *rZ,
+            double *iZ, int dim, int dim1, int dim2, int block_cnt, int thread_cnt) {
+  if (dim > 0 && (dim1 != dim + 1 || dim2 != dim * dim))
+    throw veig::Error(VEIG_ERR_INVALID, "zlahqr: dim1 must be dim + 1 and dim2 dim * dim");
+  if (dim <= 0 || block_cnt <= 0 || thread_cnt <= 0) {
+    check(veig_zlahqr_roots(dim, block_cnt, thread_cnt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            nullptr, nullptr, nullptr, 30, nullptr),
+          "zlahqr");
+    return;
+  }
+  const size_t sm = msize(dim, block_cnt, thread_cnt), sv = vsize(dim, block_cnt, thread_cnt);
+  const size_t sc = nmat(block_cnt, thread_cnt) * (size_t)(dim + 1);
+  Dev c1(rC, sc), c2(iC, sc), v1(nullptr, sm), v2(nullptr, sm), z1(nullptr, sm), z2(nullptr, sm);
+  Dev e1(nullptr, sv), e2(nullptr, sv);
+  check(veig_zlahqr_roots(dim, block_cnt, thread_cnt, c1.d, c2.d, e1.d, e2.d, v1.d, v2.d, z1.d, z2.d, nullptr,
+                          30, nullptr),
+        "zlahqr");
+  sync();
+  e1.back(rE); e2.back(iE); v1.back(rV); v2.back(iV); z1.back(rZ); z2.back(iZ);
 }
 
 void test_ident(double *rZ, double *iZ, int dim, int block_cnt, int thread_cnt) {

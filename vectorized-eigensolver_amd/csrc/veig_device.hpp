@@ -353,7 +353,10 @@ __device__ __forceinline__ void larfg2(cx<R> a0, cx<R> a1, cx<R>& t1, cx<R>& v2,
 // <= i3, and columns K, K+1 of the Schur vectors.  KC >= 0 is K as a
 // compile-time constant (unrolled sweeps); KC < 0 takes the wavefront-uniform
 // runtime Krt and selects the register columns by a uniform branch tree.
-template <int KC, typename R, int NB, int P>
+// TZ = false is the eigenvalues-only sweep (single_shift_qr, Toolbox.hpp:
+// 372-522): H is updated inside the active block L..I only (columns <= I,
+// rows >= L) and there are no Schur vectors.
+template <int KC, bool TZ = true, typename R, int NB, int P>
 __device__ __forceinline__ void sweep_step(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], bool act,
                                            int L, int I, cx<R> a0, cx<R> a1, int Krt = 0) {
   using L2 = typename Group<R, NB, P>::L2;
@@ -377,7 +380,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P>& g, cx<R> (&z)[
   for (int p = 0; p < P; ++p) {
     if ((p + 1) * LG - 1 >= K) {
       const int col = g.idx(p);
-      if (act && col >= K && col < n) {
+      if (act && col >= K && col < n && (TZ || col <= I)) {
         L2* p1 = g.at(K, col);
         L2* p2 = g.at(K + 1, col);
         cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
@@ -396,7 +399,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P>& g, cx<R> (&z)[
   for (int p = 0; p < P; ++p) {
     if (p * LG <= K + 2) {
       const int row = g.idx(p);
-      if (act && row <= i3) {
+      if (act && row <= i3 && (TZ || row >= L)) {
         L2* p1 = g.at(row, K);
         L2* p2 = g.at(row, K + 1);
         cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
@@ -409,7 +412,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P>& g, cx<R> (&z)[
     }
   }
   // Schur vectors: columns K, K+1 of each owned row
-  if (act) {
+  if (TZ && act) {
     auto upd = [&](auto kc) {
       constexpr int KK = decltype(kc)::value;
 #pragma unroll
@@ -419,18 +422,22 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P>& g, cx<R> (&z)[
         z[p][KK + 1] = sub(z[p][KK + 1], mulc(sum, v2));
       }
     };
-    if constexpr (KC >= 0) upd(std::integral_constant<int, KC>{});
-    else uniform_dispatch<0, NB - 1>(K, upd);
+    if constexpr (!TZ) {
+    } else if constexpr (KC >= 0) {
+      upd(std::integral_constant<int, KC>{});
+    } else {
+      uniform_dispatch<0, NB - 1>(K, upd);
+    }
   }
   wave_sync();
 }
 
-template <int K, int KEND, typename R, int NB, int P>
+template <int K, int KEND, bool TZ = true, typename R, int NB, int P>
 __device__ __forceinline__ void sweep_range(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], bool ready,
                                             int L, int I, cx<R> a0, cx<R> a1, int kmin, int kmax) {
   if constexpr (K < KEND) {
-    if (K >= kmin && K < kmax) sweep_step<K>(g, z, ready && K >= L && K < I, L, I, a0, a1);
-    sweep_range<K + 1, KEND>(g, z, ready, L, I, a0, a1, kmin, kmax);
+    if (K >= kmin && K < kmax) sweep_step<K, TZ>(g, z, ready && K >= L && K < I, L, I, a0, a1);
+    sweep_range<K + 1, KEND, TZ>(g, z, ready, L, I, a0, a1, kmin, kmax);
   }
 }
 
@@ -445,8 +452,11 @@ __device__ __forceinline__ void sweep_range(const Group<R, NB, P>& g, cx<R> (&z)
 // the count of such eigenvalues is returned.
 // If make_real_subdiag, the subdiagonal is first made real (loop 20,
 // Toolbox.hpp:735-793); after gehrd_q it already is.
+// TZ = false: the eigenvalues-only iteration (zlahqr_, Toolbox.hpp:1663-1890):
+// no Schur vectors and H updated inside the active block only; the active
+// block evolves exactly as with TZ = true, so the eigenvalues are the same.
 // --------------------------------------------------------------------------
-template <typename R, int NB, int P>
+template <typename R, int NB, int P, bool TZ = true>
 __device__ __forceinline__ int lahqr(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], int itmax,
                                      bool make_real_subdiag) {
   constexpr int LG = Group<R, NB, P>::LG;
@@ -479,8 +489,10 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P>& g, cx<R> (&z)[P][NB]
           const int x = g.idx(p);
           if (x <= i + 1 && x < n) st<R>(g.at(x, i), mulc(ld<R>(g.at(x, i)), sc));
         }
+        if constexpr (TZ) {
 #pragma unroll
-        for (int p = 0; p < P; ++p) z[p][i] = mulc(z[p][i], sc);
+          for (int p = 0; p < P; ++p) z[p][i] = mulc(z[p][i], sc);
+        }
       }
       wave_sync();
     }
@@ -610,9 +622,9 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P>& g, cx<R> (&z)[P][NB]
       // one matrix per wavefront: a runtime loop over K, with the Schur
       // vector columns selected by a uniform branch tree
       static_assert(P == 1, "one matrix per wavefront uses one index per lane");
-      for (int K = kmin; K < kmax; ++K) sweep_step<-1>(g, z, ready && K >= L && K < I, L, I, a0, a1, K);
+      for (int K = kmin; K < kmax; ++K) sweep_step<-1, TZ>(g, z, ready && K >= L && K < I, L, I, a0, a1, K);
     } else {
-      sweep_range<0, NB - 1>(g, z, ready, L, I, a0, a1, kmin, kmax);
+      sweep_range<0, NB - 1, TZ>(g, z, ready, L, I, a0, a1, kmin, kmax);
     }
 
     // ---- make H(I, I-1) real (Toolbox.hpp:337-367) ----
@@ -626,10 +638,11 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P>& g, cx<R> (&z)[P][NB]
 #pragma unroll
         for (int p = 0; p < P; ++p) {
           const int x = g.idx(p);
-          if (x > I && x < n) st<R>(g.at(I, x), mul(ld<R>(g.at(I, x)), zz));
-          if (x < I) st<R>(g.at(x, I), mulc(ld<R>(g.at(x, I)), zz));
+          if (TZ && x > I && x < n) st<R>(g.at(I, x), mul(ld<R>(g.at(I, x)), zz));
+          if (x < I && (TZ || x >= L)) st<R>(g.at(x, I), mulc(ld<R>(g.at(x, I)), zz));
         }
-        if constexpr (MPW == 1) {
+        if constexpr (!TZ) {
+        } else if constexpr (MPW == 1) {
           uniform_dispatch<0, NB>(I, [&](auto kc) {
             constexpr int KK = decltype(kc)::value;
             z[0][KK] = mulc(z[0][KK], zz);

@@ -56,6 +56,10 @@ _SIGS = {
     "veig_zgemv": ([_i, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                     ctypes.c_double, _vp], _i),
     "veig_backsubstitution": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp], _i),
+    "veig_zlahqr_eig": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp], _i),
+    "veig_eigvals": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp], _i),
+    "veig_zlahqr_roots": ([_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                           ctypes.c_int32, _vp], _i),
     "veig_rearrange_matrices": ([_i64, _i64, _i64, _vp, _vp, _vp], _i),
     "veig_arrange_matrices_aos": ([_i64, _i64, _i64, _vp, _vp, _vp], _i),
     "veig_rep_matrix_soa": ([_i64, _i64, _i64, _vp, _vp, _vp], _i),
@@ -167,6 +171,59 @@ def zgeev(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, info=None
         raise ValueError(precision)
     check(st, "veig_zgeev")
     return rE, iE, rV, iV
+
+
+def eigvals(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, info=None,
+            itmax: int = 30, stream=None):
+    """Eigenvalues only of general matrices (veig_eigvals).  Returns (rE, iE)."""
+    torch = _torch()
+    nm = block_cnt * thread_cnt
+    if out is None:
+        rE = torch.empty(nm * dim, dtype=torch.float64, device=rA.device)
+        iE = torch.empty_like(rE)
+    else:
+        rE, iE = out
+    st = lib().veig_eigvals(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
+                            _ptr(info), itmax, _stream(stream))
+    check(st, "veig_eigvals")
+    return rE, iE
+
+
+def zlahqr_eig(rH, iH, dim: int, block_cnt: int, thread_cnt: int, out=None, info=None,
+               itmax: int = 30, stream=None):
+    """The reference's eigenvalues-only zlahqr_ on Hessenberg matrices
+    (veig_zlahqr_eig; H updated in place).  Returns (rE, iE)."""
+    torch = _torch()
+    nm = block_cnt * thread_cnt
+    if out is None:
+        rE = torch.empty(nm * dim, dtype=torch.float64, device=rH.device)
+        iE = torch.empty_like(rE)
+    else:
+        rE, iE = out
+    st = lib().veig_zlahqr_eig(dim, block_cnt, thread_cnt, _ptr(rE), _ptr(iE), _ptr(rH), _ptr(iH),
+                               _ptr(info), itmax, _stream(stream))
+    check(st, "veig_zlahqr_eig")
+    return rE, iE
+
+
+def roots(rC, iC, dim: int, block_cnt: int, thread_cnt: int, schur: bool = False, info=None,
+          itmax: int = 30, stream=None):
+    """Roots of the polynomials c_0 + ... + c_dim x^dim (coefficients in the
+    vector layout, dim+1 per polynomial) via their companion matrices
+    (veig_zlahqr_roots).  Returns (rE, iE) or, with schur, (rE, iE, rT, iT,
+    rZ, iZ)."""
+    torch = _torch()
+    nm = block_cnt * thread_cnt
+    dev = rC.device
+    rE = torch.empty(nm * dim, dtype=torch.float64, device=dev)
+    iE = torch.empty_like(rE)
+    mats = [None] * 4
+    if schur:
+        mats = [torch.empty(nm * dim * dim, dtype=torch.float64, device=dev) for _ in range(4)]
+    st = lib().veig_zlahqr_roots(dim, block_cnt, thread_cnt, _ptr(rC), _ptr(iC), _ptr(rE), _ptr(iE),
+                                 *[_ptr(m) for m in mats], _ptr(info), itmax, _stream(stream))
+    check(st, "veig_zlahqr_roots")
+    return (rE, iE, *mats) if schur else (rE, iE)
 
 
 def mixed_workspace(dim: int, block_cnt: int, thread_cnt: int, device="cuda"):
