@@ -45,6 +45,14 @@ def nominal_flops(n: int, sweeps_per_eig: float = 2.0) -> float:
     return gehrd + unghr + trevc + qr
 
 
+def nominal_flops_eig(n: int, sweeps_per_eig: float = 2.0) -> float:
+    """Eigenvalues-only QR (no Hessenberg reduction: companion input): the
+    bulge step in an active block of order m touches about m + 3 entries of
+    each of two rows/columns, 20 flops each, over s (m - 1) steps per
+    deflation, m = n .. 2."""
+    return sum(20.0 * (m + 3) * sweeps_per_eig * (m - 1) for m in range(2, n + 1))
+
+
 def alg_bytes(n: int) -> int:
     """Read A, write V and E (complex128)."""
     return 32 * n * n + 16 * n
@@ -90,7 +98,7 @@ def roofline(flops_per_launch, bytes_per_launch, kernel_ms, kernel, traffic_key)
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "roots"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=0, help="override the configuration's batch size")
@@ -121,6 +129,8 @@ def main():
     stream = torch.cuda.current_stream()
     if a.config == "c3":
         out = bench_varsize(a, cfg, veig, W, torch, dev, stream)
+    elif a.config == "roots":
+        out = bench_roots(a, cfg, veig, W, torch, dev, stream)
     else:
         out = bench_uniform(a, cfg, veig, W, torch, dev, stream)
     print(json.dumps(out), flush=True)
@@ -313,6 +323,110 @@ def cpu_baseline_uniform(rA, iA, n, T, N, target_s):
     return {"value": round(S / dt, 1), "unit": "matrices/s", "cores": cores, "kind": kind,
             "sample": f"first {S} matrices of the same batch (tile {T}), four stages timed once "
                       f"after a warm-up call, {dt:.1f} s", "cpu": _cpu_model()}
+
+
+def bench_roots(a, cfg, veig, W, torch, dev, stream):
+    """Polynomial roots through companion matrices (veig_zlahqr_roots):
+    roots only (the eigenvalues-only iteration) is `value`; the Schur form
+    and vectors, which the reference's kernel also returns, are timed beside."""
+    import numpy as np
+    n, N = cfg["n"], cfg["batch"]
+    T = min(a.tile or 64, N)
+    bc = N // T
+    progress(f"roots: generating {N} polynomials of degree {n}")
+    rC, iC = W.coeff_soa(N, n, T, cfg["seed"], dev)
+    f64 = dict(dtype=torch.float64, device=dev)
+    rE, iE = torch.empty(N * n, **f64), torch.empty(N * n, **f64)
+    info = torch.empty(N, dtype=torch.int32, device=dev)
+    L = veig.lib()
+
+    def step_roots():
+        veig.check(L.veig_zlahqr_roots(n, bc, T, veig._ptr(rC), veig._ptr(iC), veig._ptr(rE), veig._ptr(iE),
+                                       None, None, None, None, veig._ptr(info), 30, veig._stream(stream)),
+                   "veig_zlahqr_roots")
+
+    progress("timing roots only")
+    ms, wall = timed(step_roots, a.steps, a.warmup, stream)
+    value = N / (ms * 1e-3)
+    # accuracy on a sample: against numpy's roots of the same coefficients
+    S = 256
+    cr = rC[: (S // T) * T * (n + 1) if S >= T else T * (n + 1)].cpu().numpy()
+    ci = iC[: cr.size].cpu().numpy()
+    Sm = cr.size // (n + 1)
+    import helpers
+    Cf = helpers.vec_from_soa(cr, ci, n + 1, T)
+    E = helpers.vec_from_soa(rE[: Sm * n].cpu().numpy(), iE[: Sm * n].cpu().numpy(), n, T)
+    worst = 0.0
+    for k in range(Sm):
+        want = np.roots(Cf[k, ::-1])
+        _, err = helpers.match_eigenvalues(want[None], E[k][None])
+        worst = max(worst, float(err[0] / max(1.0, np.abs(want).max())))
+    schur = None
+    if not a.no_mixed:
+        mats = [torch.empty(N * n * n, **f64) for _ in range(4)]
+
+        def step_schur():
+            veig.check(L.veig_zlahqr_roots(n, bc, T, veig._ptr(rC), veig._ptr(iC), veig._ptr(rE),
+                                           veig._ptr(iE), *[veig._ptr(m) for m in mats], veig._ptr(info), 30,
+                                           veig._stream(stream)), "veig_zlahqr_roots")
+
+        progress("timing roots with Schur form and vectors")
+        ms2, _ = timed(step_schur, a.steps, a.warmup, stream)
+        schur = {"value": round(N / (ms2 * 1e-3), 1), "ms_per_step": round(ms2, 4)}
+        del mats
+    flops = N * nominal_flops_eig(n)
+    nbytes = N * (16 * (n + 1) + 16 * n)
+    rf = roofline(flops, nbytes, ms, f"k_eig<{nb(n)}, 1, SRC_POLY, false>", f"roots_n{n}")
+    rf["flops_per_matrix"] = nominal_flops_eig(n)
+    cpu = None
+    if not a.no_cpu:
+        progress("timing the reference CPU roots kernel on a sample")
+        cpu = cpu_baseline_roots(rC, iC, n, T, N, a.cpu_seconds)
+    return {
+        "metric": f"polynomial roots/s, degree {n}, complex128 (companion matrix QR)",
+        "value": round(value, 1), "unit": "polynomials/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{N} polynomials c_0..c_{n}, re/im iid N(0,1) seed {cfg['seed']}, "
+                               f"vector layout tile {T}; roots only", "name": "roots", "batch": N, "n": n},
+        "roofline": rf, "cpu_baseline": cpu,
+        "roots": {"value": round(value, 1), "max_rel_root_err_vs_numpy": worst, "sample": Sm,
+                  "unconverged": int((info != 0).sum())},
+        "roots_with_schur": schur, "wall_s_timed": round(wall, 3)}
+
+
+def cpu_baseline_roots(rC, iC, n, T, N, target_s):
+    """The reference's roots kernel (oracle/_ref/libref.so; companion + Schur
+    form and vectors, OpenMP) on a bounded sample; the oracle restatement
+    where the reference build is absent."""
+    import numpy as np
+    import helpers
+    lib, kind = _cpu_lib()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def run(S):
+        bcs = S // T
+        c = rC[: bcs * T * (n + 1)].cpu().numpy().copy()
+        d = iC[: bcs * T * (n + 1)].cpu().numpy().copy()
+        out = [np.zeros(S * n), np.zeros(S * n)] + [np.zeros(S * n * n) for _ in range(4)]
+        P = helpers.dptr
+        t = time.perf_counter()
+        if kind == "reference":
+            lib.ref_zlahqr_roots(n, bcs, T, P(c), P(d), *[P(o) for o in out])
+        else:
+            lib.oracle_zlahqr_roots(n, bcs, T, P(c), P(d), *[P(o) for o in out], 0, None)
+        return time.perf_counter() - t
+
+    S0 = max(T, (min(N, 4096) // T) * T)
+    run(S0)
+    t0 = run(S0)
+    S = int(min(N, max(S0, target_s / max(t0, 1e-6) * S0)))
+    S = max(T, (S // T) * T)
+    dt = run(S)
+    return {"value": round(S / dt, 1), "unit": "polynomials/s", "cores": cores, "kind": kind,
+            "sample": f"first {S} polynomials of the same batch (tile {T}), the reference's zlahqr roots "
+                      f"kernel (Schur form and vectors included) timed once after a warm-up call, {dt:.1f} s",
+            "cpu": _cpu_model()}
 
 
 def bench_varsize(a, cfg, veig, W, torch, dev, stream):

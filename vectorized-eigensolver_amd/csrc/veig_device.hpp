@@ -187,20 +187,32 @@ __device__ __forceinline__ void uniform_dispatch(int k, F&& f) {
 // one lane per index; P = 2 and 4 put 2 or 4 times as many matrices in a
 // wavefront, which spreads the per-step scalar work (reflector generation,
 // shifts, deflation bookkeeping) over more matrices per instruction.
-template <typename R, int NB, int P = 1>
+// LAY = LAY_DENSE: the NB x NB matrix in an NB x (NB+1) column-major tile.
+// LAY = LAY_BAND: only the band row <= col + 2 (the Hessenberg matrix and the
+// bulge of the QR sweep), column c holding rows 0..min(c+2, NB-1) from
+// offset c(c+5)/2: 166 instead of 272 entries at NB = 16.  Enough for the
+// eigenvalues-only QR iteration, which never leaves that band.
+enum { LAY_DENSE = 0, LAY_BAND = 1 };
+
+template <typename R, int NB, int P = 1, int LAY = LAY_DENSE>
 struct Group {
   static_assert(NB % P == 0 && 64 % (NB / P) == 0, "lane group must divide the wavefront");
   static constexpr int LG = NB / P;    // lanes per matrix
   static constexpr int LD = NB + 1;
   static constexpr int MPW = 64 / LG;  // matrices per wavefront
+  // complex entries of one matrix's tile
+  static constexpr int kSize = LAY == LAY_BAND ? (NB - 1) * (NB + 4) / 2 + NB : NB * LD;
   using L2 = typename lds2<R>::type;
-  L2* H;     // NB x LD complex tile, H(r,c) at H[c*LD + r]
+  L2* H;     // the tile: H(r,c) at H[c*LD + r], or banded (see above)
   L2* vsh;   // NB complex scratch (reflector vector)
   int r;     // lane index within the group (slot 0 index)
   int mi;    // group index within the wavefront
   int n;     // order of this group's matrix (0 if the slot is empty)
   __device__ __forceinline__ int idx(int p) const { return r + p * LG; }
-  __device__ __forceinline__ L2* at(int row, int col) const { return H + col * LD + row; }
+  __device__ __forceinline__ L2* at(int row, int col) const {
+    if constexpr (LAY == LAY_BAND) return H + ((col * (col + 5)) >> 1) + row;
+    else return H + col * LD + row;
+  }
 };
 
 // --------------------------------------------------------------------------
@@ -356,11 +368,11 @@ __device__ __forceinline__ void larfg2(cx<R> a0, cx<R> a1, cx<R>& t1, cx<R>& v2,
 // TZ = false is the eigenvalues-only sweep (single_shift_qr, Toolbox.hpp:
 // 372-522): H is updated inside the active block L..I only (columns <= I,
 // rows >= L) and there are no Schur vectors.
-template <int KC, bool TZ = true, typename R, int NB, int P>
-__device__ __forceinline__ void sweep_step(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], bool act,
+template <int KC, bool TZ = true, typename R, int NB, int P, int LAY>
+__device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY>& g, cx<R> (&z)[P][NB], bool act,
                                            int L, int I, cx<R> a0, cx<R> a1, int Krt = 0) {
-  using L2 = typename Group<R, NB, P>::L2;
-  constexpr int LG = Group<R, NB, P>::LG;
+  using L2 = typename Group<R, NB, P, LAY>::L2;
+  constexpr int LG = Group<R, NB, P, LAY>::LG;
   const int n = g.n;
   const int K = KC >= 0 ? KC : Krt;
   cx<R> b0 = a0, b1 = a1;
@@ -432,8 +444,8 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P>& g, cx<R> (&z)[
   wave_sync();
 }
 
-template <int K, int KEND, bool TZ = true, typename R, int NB, int P>
-__device__ __forceinline__ void sweep_range(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], bool ready,
+template <int K, int KEND, bool TZ = true, typename R, int NB, int P, int LAY>
+__device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY>& g, cx<R> (&z)[P][NB], bool ready,
                                             int L, int I, cx<R> a0, cx<R> a1, int kmin, int kmax) {
   if constexpr (K < KEND) {
     if (K >= kmin && K < kmax) sweep_step<K, TZ>(g, z, ready && K >= L && K < I, L, I, a0, a1);
@@ -456,11 +468,12 @@ __device__ __forceinline__ void sweep_range(const Group<R, NB, P>& g, cx<R> (&z)
 // no Schur vectors and H updated inside the active block only; the active
 // block evolves exactly as with TZ = true, so the eigenvalues are the same.
 // --------------------------------------------------------------------------
-template <typename R, int NB, int P, bool TZ = true>
-__device__ __forceinline__ int lahqr(const Group<R, NB, P>& g, cx<R> (&z)[P][NB], int itmax,
+template <typename R, int NB, int P, bool TZ = true, int LAY = LAY_DENSE>
+__device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY>& g, cx<R> (&z)[P][NB], int itmax,
                                      bool make_real_subdiag) {
-  constexpr int LG = Group<R, NB, P>::LG;
-  constexpr int MPW = Group<R, NB, P>::MPW;
+  static_assert(LAY == LAY_DENSE || !TZ, "the banded tile holds no Schur form outside the band");
+  constexpr int LG = Group<R, NB, P, LAY>::LG;
+  constexpr int MPW = Group<R, NB, P, LAY>::MPW;
   const int n = g.n;
   const R ulp = limits<R>::eps;
   const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);

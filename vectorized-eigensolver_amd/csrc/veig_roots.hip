@@ -41,15 +41,16 @@ struct EigArgs {
 // Row idx(p) of the companion matrix of c_0 + c_1 x + ... + c_n x^n into
 // the tile: 1 on the subdiagonal, -c_r / c_n in the last column (Smith's
 // division, as create_companion_matrices).  The tile is zero elsewhere.
-template <int NB, int P>
-__device__ __forceinline__ void companion_rows(const Group<double, NB, P> &g, const EigArgs &a, int64_t m) {
+template <typename G>
+__device__ __forceinline__ void companion_rows(const G &g, const EigArgs &a, int64_t m) {
   const int n = g.n;
   if (n <= 0) return;
   const int64_t T = a.G.T, b = m / T, t = m - b * T;
   const int64_t cbase = b * T * (int64_t)(n + 1) + t;  // vector slot of length n + 1
   const cx<double> lead = mk<double>(a.rA[cbase + (int64_t)n * T], a.iA[cbase + (int64_t)n * T]);
+  constexpr int kSlots = (G::LD - 1) / G::LG;  // indices per lane
 #pragma unroll
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < kSlots; ++p) {
     const int r = g.idx(p);
     if (r < n) {
       const cx<double> c = mk<double>(a.rA[cbase + (int64_t)r * T], a.iA[cbase + (int64_t)r * T]);
@@ -60,35 +61,92 @@ __device__ __forceinline__ void companion_rows(const Group<double, NB, P> &g, co
   }
 }
 
+// LDS of a wavefront's MPW matrices in layout LAY (plus the reflector
+// vectors of the Hessenberg reduction for the dense layout)
+template <int NB, int P, int LAY>
+struct EigTiles {
+  using G = Group<double, NB, P, LAY>;
+  static constexpr int kTile = G::kSize + 1;  // +1: each group starts on another bank
+  static constexpr int kVec = LAY == LAY_DENSE ? NB + 1 : 0;
+  double2 H[G::MPW * kTile];
+  double2 v[G::MPW * kVec + 1];
+};
+
 template <int NB, int P, int SRC, bool TZ>
 __global__ __launch_bounds__(64) void k_eig(EigArgs a) {
   using R = double;
-  using G = Group<R, NB, P>;
-  __shared__ Tiles<R, NB, P> sm;
+  // the banded tile serves the iterations that never leave the band: no
+  // Hessenberg reduction (dense trailing matrix) and no Schur form output
+  constexpr int LAY = (SRC != SRC_MATRIX && !TZ) ? LAY_BAND : LAY_DENSE;
+  using G = Group<R, NB, P, LAY>;
+  using Tl = EigTiles<NB, P, LAY>;
+  __shared__ Tl sm;
   constexpr int MPW = G::MPW;
   const int64_t m = xcd_block() * MPW + threadIdx.x / G::LG;
   const bool valid = m < a.G.nmat;
   if (!__any(valid)) return;
   const int n = valid ? a.G.n : 0, nmax = a.G.n;
-  G g = make_group<R, NB, P>(sm, n);
+  G g;
+  g.mi = threadIdx.x / G::LG;
+  g.r = threadIdx.x % G::LG;
+  g.n = n;
+  g.H = sm.H + g.mi * Tl::kTile;
+  g.vsh = sm.v + g.mi * Tl::kVec;
+  for (int i = g.r; i < G::kSize; i += G::LG) st<R>(g.H + i, mk<R>(0, 0));
+  if constexpr (LAY == LAY_DENSE) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) st<R>(g.vsh + g.idx(p), mk<R>(0, 0));
+  }
   const int64_t base = valid ? a.G.mat_base(m) : 0, vbase = valid ? a.G.vec_base(m) : 0;
   wave_sync();
   if constexpr (SRC == SRC_POLY) {
-    companion_rows<NB, P>(g, a, m);
+    companion_rows(g, a, m);
   } else {
-    load_rows<R, NB, P>(g, a.G, base, a.rA, a.iA, SRC == SRC_HESSENBERG ? 1 : NB);
+    // rows idx(p): entries with row <= col + band (the whole row for a
+    // general matrix, the Hessenberg part otherwise)
+    constexpr int band = SRC == SRC_HESSENBERG ? 1 : NB;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int row = g.idx(p);
+      if (row < n)
+        for (int c = row > band ? row - band : 0; c < n; ++c) {
+          const int64_t idx = base + a.G.el(row, c);
+          st<R>(g.at(row, c), mk<R>(a.rA[idx], a.iA[idx]));
+        }
+    }
   }
   cx<R> z[P][NB];
   if constexpr (TZ) identity_rows<R, NB, P>(g, z);
   wave_sync();
   if constexpr (SRC == SRC_MATRIX) gehrd_q<R, NB, P>(g, z, nmax, false, false, nullptr);
   const int nfail = lahqr<R, NB, P, TZ>(g, z, a.itmax, false);
-  cx<R> lam[P];
 #pragma unroll
-  for (int p = 0; p < P; ++p) lam[p] = g.idx(p) < n ? ld<R>(g.at(g.idx(p), g.idx(p))) : mk<R>(0, 0);
-  store_eigenvalues<R, NB, P>(g, a.G, vbase, a.rE, a.iE, lam);
-  if constexpr (SRC == SRC_HESSENBERG || (SRC == SRC_POLY && TZ)) store_rows<R, NB, P>(g, a.G, base, a.rH, a.iH);
-  if constexpr (TZ) store_row_regs<R, NB, P>(g, a.G, base, a.rZ, a.iZ, z);
+  for (int p = 0; p < P; ++p) {
+    const int k = g.idx(p);
+    if (k < n) {
+      const cx<R> lam = ld<R>(g.at(k, k));
+      a.rE[vbase + a.G.ve(k)] = lam.re;
+      a.iE[vbase + a.G.ve(k)] = lam.im;
+    }
+  }
+  if constexpr (SRC == SRC_HESSENBERG) {
+    // the Hessenberg part back (entries below the subdiagonal untouched)
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int row = g.idx(p);
+      if (row < n)
+        for (int c = row > 1 ? row - 1 : 0; c < n; ++c) {
+          const int64_t idx = base + a.G.el(row, c);
+          const cx<R> h = ld<R>(g.at(row, c));
+          a.rH[idx] = h.re;
+          a.iH[idx] = h.im;
+        }
+    }
+  }
+  if constexpr (SRC == SRC_POLY && TZ) {
+    store_rows<R, NB, P>(g, a.G, base, a.rH, a.iH);
+    store_row_regs<R, NB, P>(g, a.G, base, a.rZ, a.iZ, z);
+  }
   if (a.info && valid && g.r == 0) a.info[m] = nfail;
 }
 

@@ -27,6 +27,9 @@ CONFIGS = {
     "c3": dict(nmin=4, nmax=32, batch=262144, seed=1003),
     "c4": dict(n=24, batch=524288, seed=1004),
     "c5": dict(n=64, batch=65536, seed=1005, tile=1),
+    # polynomial roots (the reference's companion-matrix kernel): degree 16,
+    # complex Gaussian coefficients c_0..c_16
+    "roots": dict(n=16, batch=1 << 20, seed=1006),
 }
 
 
@@ -44,6 +47,15 @@ def gaussian_soa(N: int, n: int, tile: int, seed: int, device="cuda"):
     rA = torch.randn(N * n * n, dtype=torch.float64, device=device, generator=g)
     iA = torch.randn(N * n * n, dtype=torch.float64, device=device, generator=g)
     return rA, iA
+
+
+def coeff_soa(N: int, n: int, tile: int, seed: int, device="cuda"):
+    """Coefficients c_0..c_n of N polynomials (re/im iid N(0,1)) in the
+    vector layout with n+1 entries per polynomial, tile `tile`."""
+    g = _gen(seed, device)
+    re = torch.randn(N * (n + 1), dtype=torch.float64, device=device, generator=g)
+    im = torch.randn(N * (n + 1), dtype=torch.float64, device=device, generator=g)
+    return re, im
 
 
 def random_unitary(N: int, n: int, g, device, reflectors: int = 4):
