@@ -192,8 +192,8 @@ int veig_zlahqr_eig(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *
                     double *rH, double *iH, int32_t *info, int32_t itmax, void *stream);
 /* Eigenvalues of general matrices: the Hessenberg reduction without Q, then
  * the iteration of veig_zlahqr_eig (A read only).  No single counterpart in
- * the reference (zgehrd followed by its zlahqr_); E is, bit for bit, the E of
- * veig_zgeev on the same input. */
+ * the reference (zgehrd followed by its zlahqr_); E is the E of veig_zgeev on
+ * the same input, to rounding. */
 int veig_eigvals(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA, const double *iA,
                  double *rE, double *iE, int32_t *info, int32_t itmax, void *stream);
 /* The polynomial-roots kernel zlahqr (Toolbox.hpp:1894-1957): for each

@@ -4,9 +4,9 @@ restatement of the reference's zlahqr_ (Toolbox.hpp:1663) and zlahqr kernel
 (Toolbox.hpp:1894), which the CPU suite pins bit for bit to the reference's
 own build.
 
-Bars: eigenvalues-only E equal, bit for bit, to the fused eigensolver's E on
-the same input for n <= 32 (the active block evolves identically in the same
-lane-group code; above 32, to 1e-12 ||A||_F); E within
+Bars: eigenvalues-only E equal to the fused eigensolver's E on the same
+input to 1e-12 ||A||_F (the active block evolves identically; the kernels are
+compiled separately and round differently, ~1e-14); E within
 1e-12 * ||H||_F of the oracle's (matched by assignment); roots within
 1e-9 * max|root| of numpy's; Schur outputs with T upper triangular,
 diag(T) = E, Z unitary to 1e-13 n and ||C Z - Z T|| <= 1e-13 n ||C||.
@@ -46,16 +46,12 @@ def test_eigvals_equal_fused_eigenvalues(gpu, n):
     rE2, iE2, _, _ = veig.zgeev(_dev(rA), _dev(iA), n, N // tc, tc)
     torch.cuda.synchronize()
     assert int((info != 0).sum()) == 0
-    if n <= 32:
-        # the same lane-group iteration with and without Schur vectors
-        assert torch.equal(rE, rE2) and torch.equal(iE, iE2)
-    else:
-        # above 32 the fused kernel is the two-wavefront one (a separately
-        # compiled copy of the iteration): equal to rounding
-        E = H.vec_from_soa(rE.cpu().numpy(), iE.cpu().numpy(), n, tc)
-        E2 = H.vec_from_soa(rE2.cpu().numpy(), iE2.cpu().numpy(), n, tc)
-        _, err = H.match_eigenvalues(E2, E)
-        assert np.all(err <= 1e-12 * np.linalg.norm(A, axis=(1, 2)))
+    # the same iteration with and without Schur vectors (the active block
+    # evolves identically; separately compiled kernels may round differently)
+    E = H.vec_from_soa(rE.cpu().numpy(), iE.cpu().numpy(), n, tc)
+    E2 = H.vec_from_soa(rE2.cpu().numpy(), iE2.cpu().numpy(), n, tc)
+    _, err = H.match_eigenvalues(E2, E)
+    assert np.all(err <= 1e-12 * np.linalg.norm(A, axis=(1, 2)))
 
 
 @pytest.mark.parametrize("n,tc", [(4, 1), (16, 4), (32, 2), (64, 1)])
@@ -119,10 +115,13 @@ def test_roots_schur_outputs(gpu, n):
     assert np.max(np.abs(np.conj(np.transpose(Z, (0, 2, 1))) @ Z - eye)) < 1e-13 * n
     nrm = np.linalg.norm(C, axis=(1, 2))
     assert np.max(np.linalg.norm(C @ Z - Z @ T, axis=(1, 2)) / nrm) < 1e-13 * n
-    # the roots-only launch gives the same roots
+    # the roots-only launch gives the same roots (to rounding: a separately
+    # compiled iteration on the same active blocks)
     rE2, iE2 = veig.roots(_dev(rC), _dev(iC), n, N // tc, tc)
     torch.cuda.synchronize()
-    assert torch.equal(rE, rE2) and torch.equal(iE, iE2)
+    E2 = H.vec_from_soa(rE2.cpu().numpy(), iE2.cpu().numpy(), n, tc)
+    _, err = H.match_eigenvalues(E, E2)
+    assert np.all(err <= 1e-12 * np.maximum(1.0, np.abs(E).max(axis=1)))
 
 
 def test_roots_known_polynomials(gpu):
