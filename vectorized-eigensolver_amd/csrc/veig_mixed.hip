@@ -103,12 +103,36 @@ __global__ void k_zero_count(int32_t *count) {
   if (threadIdx.x == 0) count[0] = 0;
 }
 
-// list of the matrices flagged for the FP64 solve (order is irrelevant:
-// every matrix is solved independently)
-__global__ void k_compact(const int32_t *flags, int64_t nmat, int32_t *list, int32_t *count) {
-  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nmat;
-       m += (int64_t)gridDim.x * blockDim.x)
-    if (flags[m]) list[atomicAdd(count, 1)] = (int32_t)m;
+// list of the matrices flagged for the FP64 solve.  Any order is correct
+// (every matrix is solved independently); each chunk of 256 consecutive
+// matrices is written as one ascending run (one atomic per chunk, ranks from
+// ballots), so that list neighbours are mostly batch neighbours and share
+// cache lines of the tiled layout when the list is walked.
+constexpr int kCompactThreads = 256;
+
+__global__ __launch_bounds__(kCompactThreads) void k_compact(const int32_t *flags, int64_t nmat, int32_t *list,
+                                                             int32_t *count) {
+  __shared__ int32_t wave_cnt[kCompactThreads / 64];
+  __shared__ int32_t chunk_base;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t c0 = (int64_t)blockIdx.x * kCompactThreads; c0 < nmat; c0 += (int64_t)gridDim.x * kCompactThreads) {
+    const int64_t m = c0 + threadIdx.x;
+    const bool f = m < nmat && flags[m] != 0;
+    const uint64_t bits = __ballot(f);
+    const int rank = __popcll(bits & ((1ull << lane) - 1));
+    if (lane == 0) wave_cnt[w] = __popcll(bits);
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int j = 0; j < kCompactThreads / 64; ++j) {
+      before += j < w ? wave_cnt[j] : 0;
+      total += wave_cnt[j];
+    }
+    if (threadIdx.x == 0) chunk_base = total ? atomicAdd(count, total) : 0;
+    __syncthreads();
+    if (f) list[chunk_base + before + rank] = (int32_t)m;
+    __syncthreads();  // wave_cnt and chunk_base are rewritten for the next chunk
+  }
 }
 
 }  // namespace
@@ -164,8 +188,9 @@ int veig_zgeev_mixed(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const d
   // the list of matrices that did not reach tol
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, w.count);
   {
-    const int64_t blocks = std::min<int64_t>((G.nmat + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(256), 0, s, w.flags, G.nmat, w.list, w.count);
+    const int64_t blocks = std::min<int64_t>((G.nmat + kCompactThreads - 1) / kCompactThreads, 4096);
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(kCompactThreads), 0, s, w.flags, G.nmat, w.list,
+                       w.count);
   }
   VEIG_HIP(hipGetLastError());
   // 3. FP64 solve of the listed matrices (persistent grid), then their backward error
