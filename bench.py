@@ -98,7 +98,7 @@ def roofline(flops_per_launch, bytes_per_launch, kernel_ms, kernel, traffic_key)
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "roots"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "roots", "sweep"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=0, help="override the configuration's batch size")
@@ -131,6 +131,8 @@ def main():
         out = bench_varsize(a, cfg, veig, W, torch, dev, stream)
     elif a.config == "roots":
         out = bench_roots(a, cfg, veig, W, torch, dev, stream)
+    elif a.config == "sweep":
+        out = bench_sweep(a, cfg, veig, W, torch, dev, stream)
     else:
         out = bench_uniform(a, cfg, veig, W, torch, dev, stream)
     print(json.dumps(out), flush=True)
@@ -393,6 +395,52 @@ def bench_roots(a, cfg, veig, W, torch, dev, stream):
         "roots": {"value": round(value, 1), "max_rel_root_err_vs_numpy": worst, "sample": Sm,
                   "unconverged": int((info != 0).sum())},
         "roots_with_schur": schur, "wall_s_timed": round(wall, 3)}
+
+
+def bench_sweep(a, cfg, veig, W, torch, dev, stream):
+    """The reference driver's sweep (zlahqr.cpp:196-252: 13,312 matrices per
+    order, one batch per order) over the orders this library takes,
+    10..64: FP64 matrices/s per order; `value` = all matrices / all time."""
+    N, T = cfg["batch"], 64
+    rates, total_ms = {}, 0.0
+    for n in range(cfg["nmin"], cfg["nmax"] + 1):
+        rA, iA = W.gaussian_soa(N, n, T, cfg["seed"] + n, dev)
+        f64 = dict(dtype=torch.float64, device=dev)
+        rE, iE = torch.empty(N * n, **f64), torch.empty(N * n, **f64)
+        rV, iV = torch.empty(N * n * n, **f64), torch.empty(N * n * n, **f64)
+        info = torch.empty(N, dtype=torch.int32, device=dev)
+        ms, _ = timed(lambda: veig.zgeev(rA, iA, n, N // T, T, out=(rE, iE, rV, iV), info=info, stream=stream),
+                      a.steps, a.warmup, stream)
+        rates[n] = round(N / (ms * 1e-3), 1)
+        total_ms += ms
+        progress(f"sweep n={n}: {rates[n]:.0f} matrices/s")
+    n_dims = len(rates)
+    value = N * n_dims / (total_ms * 1e-3)
+    cpu = None
+    if not a.no_cpu:
+        # the reference's own OpenMP code on the same 13,312-matrix batches at
+        # four of the orders (whole batch, timed once after a warm-up call)
+        lib, kind = _cpu_lib()
+        per = {}
+        for n in (16, 32, 48, 64):
+            if not cfg["nmin"] <= n <= cfg["nmax"]:
+                continue
+            rA, iA = W.gaussian_soa(N, n, T, cfg["seed"] + n, dev)
+            a_np, b_np = rA.cpu().numpy(), iA.cpu().numpy()
+            _run_cpu(lib, kind, n, T, a_np[: T * n * n].copy(), b_np[: T * n * n].copy())
+            dt = _run_cpu(lib, kind, n, T, a_np.copy(), b_np.copy())
+            per[n] = round(N / dt, 1)
+            progress(f"sweep CPU reference n={n}: {per[n]:.0f} matrices/s")
+        cpu = {"per_order": per, "unit": "matrices/s", "kind": kind,
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+               "sample": f"the full {N}-matrix batch at orders {sorted(per)}", "cpu": _cpu_model()}
+    return {"metric": f"eigendecomp matrices/s, orders {cfg['nmin']}..{cfg['nmax']}, {N} matrices per order",
+            "value": round(value, 1), "unit": "matrices/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": round(total_ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{N} complex128 Gaussian matrices per order, tile {T}, one launch per order "
+                                   f"(the reference driver's sweep sizes)", "name": "sweep", "batch": N},
+            "roofline": None, "cpu_baseline": cpu, "per_order": rates}
 
 
 def cpu_baseline_roots(rC, iC, n, T, N, target_s):

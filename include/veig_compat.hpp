@@ -66,6 +66,14 @@ void eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA,
 // C[coeff + line*n_coeffs + package*matrices*n_coeffs].  Throws veig::Error
 // if a file cannot be opened (the reference reads nothing and goes on).
 void loadPackages(int packages, int matrices, int n_coeffs, ComplexDouble *C);
+// raw binary readers (utils.cpp:52-83): dim*dim (readMatrix), dim
+// (readVector) or dim*dim*n_matrices (readMatricesFromFile) native doubles
+// from `file` into `array`.  Sizes are 64-bit (the reference's int byte counts
+// overflow at 2^31); a file that cannot be opened or holds fewer values
+// throws veig::Error (the reference leaves the array partly unread).
+void readMatrix(int dim, std::string file, double *array);
+void readVector(int dim, std::string file, double *array);
+void readMatricesFromFile(int dim, int n_matrices, double *array, std::string file);
 
 // batch entry points (Toolbox-def.hpp:8-22).  The work arrays are accepted
 // and unused: the kernels keep their work on chip.

@@ -139,6 +139,29 @@ void loadPackages(int packages, int matrices, int n_coeffs, ComplexDouble *C) {
   }
 }
 
+namespace {
+void read_doubles(const std::string &file, size_t count, double *array, const char *what) {
+  std::ifstream in(file, std::ios::binary);
+  if (!in) throw veig::Error(VEIG_ERR_INVALID, std::string(what) + ": cannot open " + file);
+  in.read(reinterpret_cast<char *>(array), (std::streamsize)(count * sizeof(double)));
+  if ((size_t)in.gcount() != count * sizeof(double))
+    throw veig::Error(VEIG_ERR_INVALID, std::string(what) + ": " + file + " holds fewer than " +
+                                           std::to_string(count) + " doubles");
+}
+}  // namespace
+
+void readMatrix(int dim, std::string file, double *array) {
+  read_doubles(file, (size_t)(dim > 0 ? dim : 0) * (dim > 0 ? dim : 0), array, "readMatrix");
+}
+
+void readVector(int dim, std::string file, double *array) {
+  read_doubles(file, (size_t)(dim > 0 ? dim : 0), array, "readVector");
+}
+
+void readMatricesFromFile(int dim, int n_matrices, double *array, std::string file) {
+  read_doubles(file, msize(dim, n_matrices, 1), array, "readMatricesFromFile");
+}
+
 void eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rE,
                  double *iE, double *rV, double *iV) {
   check(veig_host_eigensolver(dim, block_cnt, thread_cnt, rA, iA, rE, iE, rV, iV), "eigensolver");

@@ -30,6 +30,8 @@ CONFIGS = {
     # polynomial roots (the reference's companion-matrix kernel): degree 16,
     # complex Gaussian coefficients c_0..c_16
     "roots": dict(n=16, batch=1 << 20, seed=1006),
+    # the reference driver's sweep (zlahqr.cpp:196-252): 13,312 matrices per order
+    "sweep": dict(nmin=10, nmax=64, batch=13312, seed=1100),
 }
 
 
