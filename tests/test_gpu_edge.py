@@ -122,3 +122,64 @@ def test_headline_size_properties(gpu):
     R = A @ V - V * E[:, None, :]
     be = torch.linalg.norm(R, dim=(1, 2)) / (torch.linalg.norm(A, dim=(1, 2)) * torch.linalg.norm(V, dim=(1, 2)))
     assert float(be.max()) < 1e-13
+
+
+@pytest.mark.parametrize("n", [40, 64])
+def test_nan_input_terminates_large_orders(gpu, n):
+    # the two-wavefront kernel: a NaN stalls only its own matrix, which the
+    # iteration cap ends (info > 0 or non-finite output); the others are exact
+    A = H.gaussian(4, n, 30 + n)
+    A[1, 3, 5] = np.nan
+    E, V, info = gpu_eig(A, 1)
+    ok = [0, 2, 3]
+    assert np.all(info[ok] == 0) and np.max(H.backward_error(A[ok], E[ok], V[ok])) < 1e-13
+    assert info[1] > 0 or not np.all(np.isfinite(E[1]))
+
+
+def test_iteration_cap_reported_order_64(gpu):
+    A = H.gaussian(8, 64, 44)
+    _, _, info = gpu_eig(A, 1, itmax=1)
+    assert np.any(info > 0)
+    _, _, info = gpu_eig(A, 1, itmax=30)
+    assert np.all(info == 0)
+
+
+@pytest.mark.parametrize("n", [16, 48])
+def test_mixed_nan_input(gpu, n):
+    # complex64 QR fails on the NaN matrix -> FP64 fallback, which reports it
+    A = H.gaussian(8, n, 50 + n)
+    A[6, 0, 1] = np.nan
+    rA, iA = H.to_soa(A, 1)
+    dev = torch.device("cuda:0")
+    info = torch.full((8,), -7, dtype=torch.int32, device=dev)
+    fb = torch.full((8,), -7, dtype=torch.int32, device=dev)
+    be = torch.empty(8, dtype=torch.float64, device=dev)
+    rE, iE, rV, iV = veig.zgeev_mixed(torch.from_numpy(rA).to(dev), torch.from_numpy(iA).to(dev), n, 8, 1,
+                                      info=info, fallback=fb, berr=be)
+    torch.cuda.synchronize()
+    info, fb, be = info.cpu().numpy(), fb.cpu().numpy(), be.cpu().numpy()
+    E = H.vec_from_soa(rE.cpu().numpy(), iE.cpu().numpy(), n, 1)
+    ok = [k for k in range(8) if k != 6]
+    assert np.all(info[ok] == 0) and np.all(be[ok] <= 1e-12)
+    assert fb[6] == 1 and (info[6] > 0 or not np.all(np.isfinite(E[6])))
+
+
+def test_roots_zero_leading_coefficient(gpu):
+    # c_n = 0: the companion matrix holds inf/NaN; its iteration ends at the
+    # cap and the other polynomials are unaffected
+    rng = np.random.default_rng(61)
+    n, N = 8, 4
+    Cf = rng.standard_normal((N, n + 1)) + 1j * rng.standard_normal((N, n + 1))
+    Cf[2, n] = 0
+    rC, iC = H.vec_to_soa(Cf, 1)
+    dev = torch.device("cuda:0")
+    info = torch.full((N,), -7, dtype=torch.int32, device=dev)
+    rE, iE = veig.roots(torch.from_numpy(rC).to(dev), torch.from_numpy(iC).to(dev), n, N, 1, info=info)
+    torch.cuda.synchronize()
+    E = H.vec_from_soa(rE.cpu().numpy(), iE.cpu().numpy(), n, 1)
+    info = info.cpu().numpy()
+    for k in (0, 1, 3):
+        want = np.roots(Cf[k, ::-1])
+        _, err = H.match_eigenvalues(want[None], E[k][None])
+        assert info[k] == 0 and err[0] < 1e-9 * max(1.0, np.abs(want).max())
+    assert info[2] > 0 or not np.all(np.isfinite(E[2]))
