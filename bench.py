@@ -187,7 +187,7 @@ def bench_uniform(a, cfg, veig, W, torch, dev, stream):
         rf = roofline(flops, nbytes, mixed["fp32_kernel_ms"], f"k_zgeev<float, {nb(n)}, 1, 0>", f"zgeev_fp32_n{n}")
     else:
         value, ms, dtype = v64, ms64, "f64"
-        kname = "k_zgeev64 (two wavefronts per matrix)" if n > 32 else f"k_zgeev<double, {nb(n)}, 1, 0>"
+        kname = "k_gehrd64 + k_qr64 (banded QR, half-row Schur vectors)" if n > 32 else f"k_zgeev<double, {nb(n)}, 1, 0>"
         rf = roofline(flops, nbytes, ms64, kname, f"zgeev_fp64_n{n}")
     rf["flops_per_matrix"] = nominal_flops(n)
 

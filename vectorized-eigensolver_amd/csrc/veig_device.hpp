@@ -826,8 +826,8 @@ __device__ __forceinline__ void backtransform(const Group<R, NB, P>& g, cx<R> (&
 // (rows < k).  At step j lanes k > j read column j of T, which lane j only
 // starts to overwrite at step j-1, so T and the solutions share the tile.
 // --------------------------------------------------------------------------
-template <typename R, int NB>
-__device__ __forceinline__ cx<R> trevc_solve_lds(const Group<R, NB, 1>& g, int nmax) {
+template <typename R, int NB, int LAY = LAY_DENSE>
+__device__ __forceinline__ cx<R> trevc_solve_lds(const Group<R, NB, 1, LAY>& g, int nmax) {
   const int r = g.r, n = g.n;
   const R ulp = limits<R>::eps;
   const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);

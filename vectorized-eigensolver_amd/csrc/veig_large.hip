@@ -51,8 +51,8 @@ struct LargeShared {
 // The Hessenberg engine's QR iteration (lahqr with one matrix per wavefront),
 // recording the Schur-vector updates instead of applying them.  Ends every
 // sweep (and the iteration) at a workgroup barrier; the last one is flagged.
-template <typename R>
-__device__ int lahqr_recorded(const Group<R, kN, 1> &g, LargeShared<R> &sh, int itmax) {
+template <typename R, int LAY, typename Shared>
+__device__ int lahqr_recorded(const Group<R, kN, 1, LAY> &g, Shared &sh, int itmax) {
   using L2 = typename lds2<R>::type;
   const int r = g.r, n = g.n;
   const R ulp = limits<R>::eps;
@@ -350,7 +350,7 @@ __device__ __forceinline__ void solve_one(const FusedArgs &a, LargeShared<R> &sh
   if (w == 0) {
     // ---------------- Hessenberg engine ----------------
     for (int i = 0; i + 1 < nmax; ++i) gehrd_step<R>(g, sh, i);
-    const int nfail = lahqr_recorded<R>(g, sh, a.itmax);
+    const int nfail = lahqr_recorded(g, sh, a.itmax);
     if (OUT == OUT_V && (a.flags & VEIG_WRITE_SCHUR)) store_rows<R, kN, 1>(g, a.G, base, a.rA, a.iA);
     const cx<R> lam = trevc_solve_lds<R, kN>(g, nmax);
     if (lane < n) {
@@ -453,6 +453,211 @@ __global__ __launch_bounds__(128) void k_zgeev64_list(FusedArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Orders 33..64 in FP64, whole batches: two kernels.
+//
+// k_gehrd64 (one wavefront per matrix, the dense tile): the Hessenberg
+// reduction alone, in zgehrd's compact form -- H on and above the
+// subdiagonal, the reflector vectors v(i+2:n) below it, tau(i) -- written to
+// the V and E planes, which serve as its workspace.
+//
+// k_qr64 (one workgroup of three wavefronts per matrix): the QR iteration and
+// the eigenvectors with only the band row <= col + 2 of H in LDS (35 KB
+// instead of 66 KB) and the Schur-vector rows split over two wavefronts, two
+// lanes per row (each lane holds half a row, 32 complex values): wavefront 0
+// is the Hessenberg engine of k_zgeev64 on the banded tile; wavefronts 1 and
+// 2 (rows 0..31, 32..63) first form Q from the reflectors (Q <- Q H_i, the
+// row's two halves combined by a lane swap), then replay each recorded sweep;
+// the reflector on columns (31, 32) pairs the two halves of a row through the
+// same swap.  About 40 KB of LDS and ~170 registers per wavefront let four
+// matrices share a CU, against two for k_zgeev64.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_gehrd64(FusedArgs a) {
+  __shared__ double2 H[kN * (kN + 1) + 1];
+  __shared__ double2 v[kN + 1];
+  const int lane = threadIdx.x;
+  const int64_t m = xcd_block();
+  if (m >= a.G.nmat) return;  // workgroup-uniform
+  const int n = a.G.n, nmax = a.G.n;
+  const int64_t base = a.G.mat_base(m), vbase = a.G.vec_base(m);
+  Group<double, kN, 1> g;
+  g.mi = 0;
+  g.r = lane;
+  g.n = n;
+  g.H = H;
+  g.vsh = v;
+#pragma unroll
+  for (int c = 0; c < kN; ++c) st<double>(g.at(lane, c), mk<double>(0, 0));
+  st<double>(g.vsh + lane, mk<double>(0, 0));
+  wave_sync();
+  load_rows<double, kN, 1>(g, a.G, base, a.rA, a.iA, kN);
+  wave_sync();
+  cx<double> z[1][kN];  // no Q here
+  cx<double> tau[1] = {mk<double>(0, 0)};
+  gehrd_q<double, kN, 1>(g, z, nmax, false, true, tau);
+  store_rows<double, kN, 1>(g, a.G, base, a.rV, a.iV);
+  if (lane + 1 < n) {
+    a.rE[vbase + a.G.ve(lane)] = tau[0].re;
+    a.iE[vbase + a.G.ve(lane)] = tau[0].im;
+  }
+}
+
+using GBand = Group<double, kN, 1, LAY_BAND>;
+
+struct QrShared64 {
+  double2 H[GBand::kSize + 1];
+  SweepLog<double> log[2];
+};
+
+// the other half of this lane's row (lane ^ 32)
+__device__ __forceinline__ cx<double> half_swap(cx<double> x) {
+  return mk<double>(__shfl_xor(x.re, 32), __shfl_xor(x.im, 32));
+}
+
+// Schur-vector engine of k_qr64: this lane's half row zh[c] = Z(row, 32 h + c).
+__device__ __forceinline__ void replay_sweeps_half(const QrShared64 &sh, int h, cx<double> (&zh)[32]) {
+  int buf = 0;
+  for (;;) {
+    __syncthreads();
+    const SweepLog<double> &lg = sh.log[buf];
+    const int L = lg.L, I = lg.I;
+#pragma unroll
+    for (int K = 0; K + 1 < kN; ++K) {
+      if (K >= L && K < I) {
+        const cx<double> t1 = ld<double>(&lg.t1[K]), v2 = ld<double>(&lg.v2[K]);
+        const double t2 = t1.re * v2.re - t1.im * v2.im;
+        if (K + 1 < 32) {
+          if (h == 0) {
+            const cx<double> sum = add(mul(t1, zh[K]), scale(zh[K + 1], t2));
+            zh[K] = sub(zh[K], sum);
+            zh[K + 1] = sub(zh[K + 1], mulc(sum, v2));
+          }
+        } else if (K >= 32) {
+          if (h == 1) {
+            const cx<double> sum = add(mul(t1, zh[K - 32]), scale(zh[K - 31], t2));
+            zh[K - 32] = sub(zh[K - 32], sum);
+            zh[K - 31] = sub(zh[K - 31], mulc(sum, v2));
+          }
+        } else {
+          // columns 31 and 32: one in each half of the row
+          const cx<double> mine = h == 0 ? zh[31] : zh[0];
+          const cx<double> other = half_swap(mine);
+          const cx<double> z31 = h == 0 ? mine : other, z32 = h == 0 ? other : mine;
+          const cx<double> sum = add(mul(t1, z31), scale(z32, t2));
+          if (h == 0) zh[31] = sub(z31, sum);
+          else zh[0] = sub(z32, mulc(sum, v2));
+        }
+      }
+    }
+    if (lg.has_phase) {
+      const cx<double> ph = ld<double>(&lg.phase);
+#pragma unroll
+      for (int c = 1; c < kN; ++c)
+        if (c == I && h == c / 32) zh[c % 32] = mulc(zh[c % 32], ph);
+    }
+    const int last = lg.last;
+    buf ^= 1;
+    if (last) break;
+  }
+}
+
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k_qr64(FusedArgs a) {
+  __shared__ QrShared64 sh;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t m = xcd_block();
+  if (m >= a.G.nmat) return;  // workgroup-uniform
+  const int n = a.G.n, nmax = a.G.n;
+  const int64_t base = a.G.mat_base(m), vbase = a.G.vec_base(m);
+  GBand g;
+  g.mi = 0;
+  g.r = lane;
+  g.n = n;
+  g.H = sh.H;
+  g.vsh = nullptr;
+  if (w == 0) {
+    // the Hessenberg part of the reduction's output, into the band
+    for (int i = lane; i < GBand::kSize; i += 64) st<double>(sh.H + i, mk<double>(0, 0));
+    wave_sync();
+    if (lane < n)
+      for (int c = lane > 0 ? lane - 1 : 0; c < n; ++c) {
+        const int64_t idx = base + a.G.el(lane, c);
+        st<double>(g.at(lane, c), mk<double>(a.rV[idx], a.iV[idx]));
+      }
+  }
+  __syncthreads();
+  if (w == 0) {
+    // ---------------- Hessenberg engine ----------------
+    const int nfail = lahqr_recorded(g, sh, a.itmax);
+    if (a.flags & VEIG_WRITE_SCHUR) {
+      if (lane < n)
+        for (int c = 0; c < n; ++c) {
+          const int64_t idx = base + a.G.el(lane, c);
+          const cx<double> t = c >= lane ? ld<double>(g.at(lane, c)) : mk<double>(0, 0);
+          a.rA[idx] = t.re;
+          a.iA[idx] = t.im;
+        }
+    }
+    const cx<double> lam = trevc_solve_lds(g, nmax);  // leaves y = -x above the diagonal
+    if (lane < n) {
+      a.rE[vbase + a.G.ve(lane)] = lam.re;
+      a.iE[vbase + a.G.ve(lane)] = lam.im;
+    }
+    if (a.info && lane == 0) a.info[m] = nfail;
+    __syncthreads();  // the eigenvector factor is ready
+  } else {
+    // ---------------- Schur-vector engines: half a row per lane ----------------
+    const int h = lane >> 5;
+    const int row = (w - 1) * 32 + (lane & 31);
+    cx<double> zh[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) zh[c] = mk<double>(32 * h + c == row ? 1.0 : 0.0, 0);
+    // Q = H_0 H_1 ... H_{n-2} from the reflectors the reduction left in V and
+    // E: lane l loads v(l) of reflector i, and each lane fetches the 32
+    // entries of its half row from the other lanes (a lane permute, no LDS),
+    // once for w = Z v and again for the update Z -= tau w v^H
+    for (int i = 0; i + 1 < nmax; ++i) {
+      cx<double> vl = mk<double>(lane == i + 1 ? 1.0 : 0.0, 0);
+      if (lane >= i + 2 && lane < n) {
+        const int64_t idx = base + a.G.el(lane, i);
+        vl = mk<double>(a.rV[idx], a.iV[idx]);
+      }
+      const cx<double> tau = mk<double>(a.rE[vbase + a.G.ve(i)], a.iE[vbase + a.G.ve(i)]);
+      int src = 32 * h;
+      asm volatile("" : "+v"(src));
+      cx<double> wv = mk<double>(0, 0);
+#pragma unroll
+      for (int c = 0; c < 32; ++c)
+        wv = add(wv, mul(zh[c], mk<double>(__shfl(vl.re, src + c), __shfl(vl.im, src + c))));
+      wv = add(wv, half_swap(wv));
+      const cx<double> tw = mul(tau, wv);
+      asm volatile("" : "+v"(src));  // permute again rather than keep 32 values live
+#pragma unroll
+      for (int c = 0; c < 32; ++c)
+        zh[c] = sub(zh[c], mulc(tw, mk<double>(__shfl(vl.re, src + c), __shfl(vl.im, src + c))));
+    }
+    replay_sweeps_half(sh, h, zh);
+    __syncthreads();  // the eigenvector factor is ready
+    // V(row, k) = Z(row, k) - sum_{j<k} Z(row, j) y(j, k): each lane sums its
+    // half of the row, the two halves are added through the lane swap, and
+    // the lanes with h = k % 2 write column k
+    for (int k = 0; k < n; ++k) {
+      cx<double> acc = mk<double>(0, 0);
+#pragma unroll
+      for (int c = 0; c < 32; ++c) {
+        const int j = 32 * h + c;
+        if (j < k) acc = add(acc, mul(zh[c], ld<double>(g.at(j, k))));
+        else if (j == k) acc = sub(acc, zh[c]);
+      }
+      acc = add(acc, half_swap(acc));
+      if (row < n && h == (k & 1)) {
+        const int64_t idx = base + a.G.el(row, k);
+        a.rV[idx] = -acc.re;
+        a.iV[idx] = -acc.im;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 namespace veig {
@@ -468,6 +673,9 @@ int launch_zgeev64(const FusedArgs &a, hipStream_t s) {
     } else {
       return fail(VEIG_ERR_UNSUPPORTED, "orders above 32: list launches are FP64 only");
     }
+  } else if constexpr (std::is_same<R, double>::value && OUT == OUT_V) {
+    hipLaunchKernelGGL(k_gehrd64, dim3((unsigned)a.G.nmat), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_qr64, dim3((unsigned)a.G.nmat), dim3(192), 0, s, a);
   } else {
     hipLaunchKernelGGL((k_zgeev64<R, OUT>), dim3((unsigned)a.G.nmat), dim3(128), 0, s, a);
   }
