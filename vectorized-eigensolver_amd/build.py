@@ -40,7 +40,8 @@ def build_lib(verbose: bool = False) -> Path:
     from concurrent.futures import ThreadPoolExecutor
 
     srcs = [CSRC / "veig.hip", CSRC / "veig_mixed.hip", CSRC / "veig_varsize.hip",
-            CSRC / "veig_large.hip", CSRC / "veig_roots.hip", CSRC / "veig_compat.cpp"]
+            CSRC / "veig_large.hip", CSRC / "veig_roots.hip", CSRC / "veig_band.hip",
+            CSRC / "veig_compat.cpp"]
     srcs = [s for s in srcs if s.exists()]
     objdir = REPO / "_build"
     objdir.mkdir(exist_ok=True)

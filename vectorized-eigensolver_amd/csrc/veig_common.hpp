@@ -414,6 +414,11 @@ inline void launch_one(const FusedArgs &a, hipStream_t s) {
 template <typename R, int OUT>
 int launch_zgeev64(const FusedArgs &a, hipStream_t s);
 
+// FP64 whole batches at 8 < n <= NB: Hessenberg reduction, then the QR
+// iteration and eigenvectors on a banded tile (two kernels)
+template <int NB>
+int launch_banded(const FusedArgs &a, hipStream_t s);
+
 template <typename R, int OUT = OUT_V>
 int launch_zgeev(const FusedArgs &a, hipStream_t s) {
   constexpr bool f32 = std::is_same<R, float>::value;
@@ -431,6 +436,9 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
           launch_one<R, 16, 4, OUT>(a, s);
           break;
         }
+      }
+      if constexpr (!f32 && OUT == OUT_V) {
+        if (p == 1 && !a.list && !a.mask) return launch_banded<16>(a, s);
       }
       if (p == 2) launch_one<R, 16, 2, OUT>(a, s);
       else launch_one<R, 16, 1, OUT>(a, s);

@@ -471,7 +471,6 @@ __device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY>& g, cx<R>
 template <typename R, int NB, int P, bool TZ = true, int LAY = LAY_DENSE>
 __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY>& g, cx<R> (&z)[P][NB], int itmax,
                                      bool make_real_subdiag) {
-  static_assert(LAY == LAY_DENSE || !TZ, "the banded tile holds no Schur form outside the band");
   constexpr int LG = Group<R, NB, P, LAY>::LG;
   constexpr int MPW = Group<R, NB, P, LAY>::MPW;
   const int n = g.n;
