@@ -113,8 +113,8 @@ def main():
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the batch path is data parallel over "
-              "independent matrices and this build runs on a single MI355X "
-              "(see the sharding note in DESIGN.md); use --gpus 1.", file=sys.stderr)
+              "independent matrices, sharded across GPUs only by splitting the batch "
+              "into contiguous tile ranges, which this library leaves to the caller; use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -187,7 +187,8 @@ def bench_uniform(a, cfg, veig, W, torch, dev, stream):
         rf = roofline(flops, nbytes, mixed["fp32_kernel_ms"], f"k_zgeev<float, {nb(n)}, 1, 0>", f"zgeev_fp32_n{n}")
     else:
         value, ms, dtype = v64, ms64, "f64"
-        kname = "k_gehrd64 + k_qr64 (banded QR, half-row Schur vectors)" if n > 32 else f"k_zgeev<double, {nb(n)}, 1, 0>"
+        kname = ("k_gehrd64 + k_qr64 (banded QR, half-row Schur vectors)" if n > 32 else
+                 "k_gehrd_c<16> + k_qr_band<16> (banded QR)" if 8 < n <= 16 else f"k_zgeev<double, {nb(n)}, 1, 0>")
         rf = roofline(flops, nbytes, ms64, kname, f"zgeev_fp64_n{n}")
     rf["flops_per_matrix"] = nominal_flops(n)
 

@@ -1,6 +1,6 @@
 """A C++ program written against the reference's host API builds against
-include/veig_compat.hpp with g++ and links libveig.so (the drop-in of
-INTEGRATION.md, section 1), and the shim's file readers (utils.cpp:17-83:
+include/veig_compat.hpp with g++ and links libveig.so (the drop-in for a
+C++ caller of the reference's host API), and the shim's file readers (utils.cpp:17-83:
 readMatrix, readVector, readMatricesFromFile, loadPackages) read what the
 reference's would.  No GPU needed: only host code runs."""
 from __future__ import annotations

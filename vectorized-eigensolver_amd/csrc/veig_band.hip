@@ -60,6 +60,9 @@ struct BandTiles {
   double2 v[G::MPW * kVec];  // the reflector being applied while Q is formed
 };
 
+// three wavefronts per SIMD: what the LDS allows at NB = 16 (at NB = 32 the
+// 32-entry Schur-vector row alone takes 128 registers and the kernel needs
+// over 256, so the fused kernel is kept there)
 template <int NB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_qr_band(FusedArgs a) {
   using G = Group<double, NB, 1, LAY_BAND>;
