@@ -28,82 +28,6 @@ using namespace veig;
 
 namespace {
 
-// The Hessenberg reduction of gehrd_q (zgehrd with zlarfg's LAPACK sign,
-// Toolbox.hpp:1315-1469) for one index per lane, leaving the compact form:
-// v(i+2:n) below the subdiagonal, tau(i) in tau_out of lane i.  Each update
-// first loads the lane's whole row (right update) or column (left update)
-// and the reflector into registers -- independent loads the hardware
-// overlaps -- instead of a runtime loop whose every iteration waits on its
-// own LDS round trip; the masked entries (c <= i) are loaded and left alone.
-template <int NB>
-__device__ __forceinline__ void gehrd_compact(const Group<double, NB, 1> &g, int nmax, cx<double> &tau_out) {
-  using R = double;
-  constexpr int LG = Group<R, NB, 1>::LG;
-  const int n = g.n, r = g.r;
-  const bool in = r < n;
-  for (int i = 0; i + 1 < nmax; ++i) {
-    // ---- reflector annihilating H(i+2:n, i) ----
-    const cx<R> hri = in ? ld<R>(g.at(r, i)) : mk<R>(0, 0);
-    const R xnorm2 = group_sum<LG, R>(r >= i + 2 && in ? abs2(hri) : R(0));
-    const bool live = i + 1 < n;
-    const cx<R> alpha = live ? ld<R>(g.at(i + 1, i)) : mk<R>(0, 0);
-    cx<R> tau = mk<R>(0, 0), scal = mk<R>(0, 0);
-    R beta = alpha.re;
-    if (live && !(xnorm2 == R(0) && alpha.im == R(0))) {
-      const R nrm = fsqrt(abs2(alpha) + xnorm2);
-      beta = alpha.re >= R(0) ? -nrm : nrm;
-      const R ib = frcp(beta);
-      tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
-      scal = inv(mk(alpha.re - beta, alpha.im));
-    }
-    cx<R> vr = mk<R>(0, 0);
-    if (live) {
-      if (r == i + 1) vr = mk<R>(1, 0);
-      else if (r >= i + 2 && in) vr = mul(hri, scal);
-    }
-    st<R>(g.vsh + r, vr);
-    if (live && in && r >= i + 1) st<R>(g.at(r, i), r == i + 1 ? mk<R>(beta, 0) : vr);
-    if (r == i) tau_out = tau;
-    wave_sync();
-    // ---- H <- H (I - tau v v^H): row r over columns i+1..n-1 ----
-    if (live) {
-      cx<R> h[NB], v[NB];
-#pragma unroll
-      for (int c = 0; c < NB; ++c) {
-        h[c] = ld<R>(g.at(r, c));
-        v[c] = ld<R>(g.vsh + c);  // zero at c <= i and beyond n
-      }
-      cx<R> w = mk<R>(0, 0);
-#pragma unroll
-      for (int c = 0; c < NB; ++c)
-        if (c > i) w = add(w, mul(h[c], v[c]));
-      w = mul(tau, w);
-#pragma unroll
-      for (int c = 0; c < NB; ++c)
-        if (c > i && c < n && in) st<R>(g.at(r, c), sub(h[c], mulc(w, v[c])));
-    }
-    wave_sync();
-    // ---- H <- (I - conj(tau) v v^H) H: column r over rows i+1..n-1 ----
-    if (live) {
-      cx<R> h[NB], v[NB];
-#pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        h[k] = ld<R>(g.at(k, r));
-        v[k] = ld<R>(g.vsh + k);
-      }
-      cx<R> y = mk<R>(0, 0);
-#pragma unroll
-      for (int k = 0; k < NB; ++k)
-        if (k > i) y = add(y, cmul(v[k], h[k]));
-      y = cmul(tau, y);  // conj(tau) * y
-#pragma unroll
-      for (int k = 0; k < NB; ++k)
-        if (k > i && k < n && r > i && in) st<R>(g.at(k, r), sub(h[k], mul(v[k], y)));
-    }
-    wave_sync();
-  }
-}
-
 template <int NB>
 __global__ __launch_bounds__(64) void k_gehrd_c(FusedArgs a) {
   using G = Group<double, NB, 1>;

@@ -336,6 +336,124 @@ __device__ __forceinline__ void gehrd_q(const Group<R, NB, P>& g, cx<R> (&z)[P][
   }
 }
 
+// The Hessenberg reduction of gehrd_q (zgehrd with zlarfg's LAPACK sign,
+// Toolbox.hpp:1315-1469) for one index per lane, leaving the compact form:
+// v(i+2:n) below the subdiagonal, tau(i) in tau_out of lane i.  Each update
+// takes the lane's row (right update) or column (left update) and the
+// reflector into registers a chunk of up to 16 entries at a time -- loads
+// the hardware overlaps -- instead of a runtime loop whose every iteration
+// waits on its own LDS round trip; entries at or left of column i are
+// loaded and left alone, chunks wholly there are skipped.  With one chunk
+// (NB <= 16) the update reuses the entries the product loaded.
+template <int NB>
+__device__ __forceinline__ void gehrd_compact(const Group<double, NB, 1> &g, int nmax, cx<double> &tau_out) {
+  using R = double;
+  constexpr int LG = Group<R, NB, 1>::LG;
+  constexpr int CH = NB < 16 ? NB : 16;  // entries per chunk
+  constexpr int NCH = NB / CH;
+  const int n = g.n, r = g.r;
+  const bool in = r < n;
+  for (int i = 0; i + 1 < nmax; ++i) {
+    // ---- reflector annihilating H(i+2:n, i) ----
+    const cx<R> hri = in ? ld<R>(g.at(r, i)) : mk<R>(0, 0);
+    const R xnorm2 = group_sum<LG, R>(r >= i + 2 && in ? abs2(hri) : R(0));
+    const bool live = i + 1 < n;
+    const cx<R> alpha = live ? ld<R>(g.at(i + 1, i)) : mk<R>(0, 0);
+    cx<R> tau = mk<R>(0, 0), scal = mk<R>(0, 0);
+    R beta = alpha.re;
+    if (live && !(xnorm2 == R(0) && alpha.im == R(0))) {
+      const R nrm = fsqrt(abs2(alpha) + xnorm2);
+      beta = alpha.re >= R(0) ? -nrm : nrm;
+      const R ib = frcp(beta);
+      tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
+      scal = inv(mk(alpha.re - beta, alpha.im));
+    }
+    cx<R> vr = mk<R>(0, 0);
+    if (live) {
+      if (r == i + 1) vr = mk<R>(1, 0);
+      else if (r >= i + 2 && in) vr = mul(hri, scal);
+    }
+    st<R>(g.vsh + r, vr);
+    if (live && in && r >= i + 1) st<R>(g.at(r, i), r == i + 1 ? mk<R>(beta, 0) : vr);
+    if (r == i) tau_out = tau;
+    wave_sync();
+    // ---- H <- H (I - tau v v^H): row r over columns i+1..n-1 ----
+    if (live) {
+      cx<R> h[CH], v[CH];
+      cx<R> w = mk<R>(0, 0);
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        if (CH * q + CH - 1 > i) {
+#pragma unroll
+          for (int j = 0; j < CH; ++j) {
+            h[j] = ld<R>(g.at(r, CH * q + j));
+            v[j] = ld<R>(g.vsh + CH * q + j);  // zero at c <= i and beyond n
+          }
+#pragma unroll
+          for (int j = 0; j < CH; ++j)
+            if (CH * q + j > i) w = add(w, mul(h[j], v[j]));
+        }
+      }
+      w = mul(tau, w);
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        if (CH * q + CH - 1 > i) {
+          if constexpr (NCH > 1) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+              h[j] = ld<R>(g.at(r, CH * q + j));
+              v[j] = ld<R>(g.vsh + CH * q + j);
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < CH; ++j) {
+            const int c = CH * q + j;
+            if (c > i && c < n && in) st<R>(g.at(r, c), sub(h[j], mulc(w, v[j])));
+          }
+        }
+      }
+    }
+    wave_sync();
+    // ---- H <- (I - conj(tau) v v^H) H: column r over rows i+1..n-1 ----
+    if (live) {
+      cx<R> h[CH], v[CH];
+      cx<R> y = mk<R>(0, 0);
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        if (CH * q + CH - 1 > i) {
+#pragma unroll
+          for (int j = 0; j < CH; ++j) {
+            h[j] = ld<R>(g.at(CH * q + j, r));
+            v[j] = ld<R>(g.vsh + CH * q + j);
+          }
+#pragma unroll
+          for (int j = 0; j < CH; ++j)
+            if (CH * q + j > i) y = add(y, cmul(v[j], h[j]));
+        }
+      }
+      y = cmul(tau, y);  // conj(tau) * y
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        if (CH * q + CH - 1 > i) {
+          if constexpr (NCH > 1) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+              h[j] = ld<R>(g.at(CH * q + j, r));
+              v[j] = ld<R>(g.vsh + CH * q + j);
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < CH; ++j) {
+            const int k = CH * q + j;
+            if (k > i && k < n && r > i && in) st<R>(g.at(k, r), sub(h[j], mul(v[j], y)));
+          }
+        }
+      }
+    }
+    wave_sync();
+  }
+}
+
 // --------------------------------------------------------------------------
 // 2-element Householder reflector of the bulge chase (zlarfg with n = 2,
 // LAPACK sign, Toolbox.hpp:183-242): given (a0, a1) returns t1, v2 and the

@@ -492,13 +492,12 @@ __global__ __launch_bounds__(64) void k_gehrd64(FusedArgs a) {
   wave_sync();
   load_rows<double, kN, 1>(g, a.G, base, a.rA, a.iA, kN);
   wave_sync();
-  cx<double> z[1][kN];  // no Q here
-  cx<double> tau[1] = {mk<double>(0, 0)};
-  gehrd_q<double, kN, 1>(g, z, nmax, false, true, tau);
+  cx<double> tau = mk<double>(0, 0);
+  gehrd_compact<kN>(g, nmax, tau);
   store_rows<double, kN, 1>(g, a.G, base, a.rV, a.iV);
   if (lane + 1 < n) {
-    a.rE[vbase + a.G.ve(lane)] = tau[0].re;
-    a.iE[vbase + a.G.ve(lane)] = tau[0].im;
+    a.rE[vbase + a.G.ve(lane)] = tau.re;
+    a.iE[vbase + a.G.ve(lane)] = tau.im;
   }
 }
 
