@@ -28,12 +28,31 @@ using namespace veig;
 
 namespace {
 
+// the matrix of this lane's group (of MPW per wavefront): from the batch,
+// or from a.list, whose *a.list_count entries the grid's first blocks take
+// -- blocks are dispatched in order, round-robin over the XCDs, so the work
+// spreads over all of them -- while the blocks past them find no matrix
+template <int MPW>
+__device__ __forceinline__ bool slot_matrix(const FusedArgs &a, int group, int64_t &m) {
+  if (a.list) {
+    const int64_t cnt = *a.list_count;
+    const int64_t used = (cnt + MPW - 1) / MPW;
+    const int64_t b = blockIdx.x;
+    const int64_t slot = b < used ? xcd_remap(b, used) * MPW + group : cnt;
+    const bool valid = slot < cnt;
+    m = valid ? a.list[slot] : 0;
+    return valid;
+  }
+  m = xcd_block() * MPW + group;
+  return m < a.G.nmat;
+}
+
 template <int NB>
 __global__ __launch_bounds__(64) void k_gehrd_c(FusedArgs a) {
   using G = Group<double, NB, 1>;
   __shared__ Tiles<double, NB, 1> sm;
-  const int64_t m = xcd_block() * G::MPW + threadIdx.x / G::LG;
-  const bool valid = m < a.G.nmat;
+  int64_t m;
+  const bool valid = slot_matrix<G::MPW>(a, threadIdx.x / G::LG, m);
   if (!__any(valid)) return;
   const int n = valid ? a.G.n : 0, nmax = a.G.n;
   G g = make_group<double, NB, 1>(sm, n, threadIdx.x);
@@ -59,17 +78,19 @@ struct BandTiles {
   double2 v[G::MPW * kVec];  // the reflector being applied while Q is formed
 };
 
-// three wavefronts per SIMD: what the LDS allows at NB = 16 (at NB = 32 the
-// 32-entry Schur-vector row alone takes 128 registers and the kernel needs
-// over 256, so the fused kernel is kept there)
+// wavefronts per SIMD the register allocator is held to: what the LDS
+// allows (11.8 KB per wavefront at NB = 16, 19.5 KB at NB = 32)
 template <int NB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_qr_band(FusedArgs a) {
+constexpr int qr_band_waves() { return NB == 16 ? 3 : 2; }
+
+template <int NB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(qr_band_waves<NB>()))) void k_qr_band(FusedArgs a) {
   using G = Group<double, NB, 1, LAY_BAND>;
   using Tl = BandTiles<NB>;
   __shared__ Tl sm;
   const int lane = threadIdx.x;
-  const int64_t m = xcd_block() * G::MPW + lane / G::LG;
-  const bool valid = m < a.G.nmat;
+  int64_t m;
+  const bool valid = slot_matrix<G::MPW>(a, lane / G::LG, m);
   if (!__any(valid)) return;
   const int n = valid ? a.G.n : 0, nmax = a.G.n;
   G g;
@@ -157,6 +178,8 @@ namespace veig {
 template <int NB>
 int launch_banded(const FusedArgs &a, hipStream_t s) {
   constexpr int mpw = Group<double, NB, 1>::MPW;
+  // a list launch gets the whole batch's grid: the list's length is only
+  // known on the device, and it never exceeds the batch
   const unsigned grid = (unsigned)((a.G.nmat + mpw - 1) / mpw);
   hipLaunchKernelGGL(k_gehrd_c<NB>, dim3(grid), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_qr_band<NB>, dim3(grid), dim3(64), 0, s, a);
@@ -164,5 +187,6 @@ int launch_banded(const FusedArgs &a, hipStream_t s) {
   return VEIG_OK;
 }
 template int launch_banded<16>(const FusedArgs &, hipStream_t);
+template int launch_banded<32>(const FusedArgs &, hipStream_t);
 
 }  // namespace veig

@@ -66,13 +66,14 @@ inline int nb_for(int64_t n) { return n <= 8 ? 8 : (n <= 16 ? 16 : (n <= 32 ? 32
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Adjacent matrices share cache lines in the tiled layout, so remap the
 // launch order so that each XCD walks one contiguous range of matrices.
-__device__ __forceinline__ int64_t xcd_block() {
+// (xcd_remap: the same over the first g blocks of a larger grid)
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t g) {
   constexpr int kXcd = 8;
-  const int64_t g = gridDim.x, b = blockIdx.x;
   const int64_t g8 = g / kXcd * kXcd;
   if (b >= g8) return b;
   return (b % kXcd) * (g8 / kXcd) + b / kXcd;
 }
+__device__ __forceinline__ int64_t xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
 
 // --------------------------------------------------------------------------
 // common kernel prologue: lane group setup and LDS tile
@@ -414,8 +415,8 @@ inline void launch_one(const FusedArgs &a, hipStream_t s) {
 template <typename R, int OUT>
 int launch_zgeev64(const FusedArgs &a, hipStream_t s);
 
-// FP64 whole batches at 8 < n <= NB: Hessenberg reduction, then the QR
-// iteration and eigenvectors on a banded tile (two kernels)
+// FP64 batches and lists at NB / 2 < n <= NB: Hessenberg reduction, then
+// the QR iteration and eigenvectors on a banded tile (two kernels)
 template <int NB>
 int launch_banded(const FusedArgs &a, hipStream_t s);
 
@@ -438,12 +439,15 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
         }
       }
       if constexpr (!f32 && OUT == OUT_V) {
-        if (p == 1 && !a.list && !a.mask) return launch_banded<16>(a, s);
+        if (p == 1 && !a.mask) return launch_banded<16>(a, s);
       }
       if (p == 2) launch_one<R, 16, 2, OUT>(a, s);
       else launch_one<R, 16, 1, OUT>(a, s);
       break;
     case 32:
+      if constexpr (!f32 && OUT == OUT_V) {
+        if (p == 1 && !a.mask) return launch_banded<32>(a, s);
+      }
       if constexpr (f32) {
         if (p == 2) {
           launch_one<R, 32, 2, OUT>(a, s);
