@@ -188,7 +188,7 @@ def bench_uniform(a, cfg, veig, W, torch, dev, stream):
     else:
         value, ms, dtype = v64, ms64, "f64"
         kname = ("k_gehrd64 + k_qr64 (banded QR, half-row Schur vectors)" if n > 32 else
-                 "k_gehrd_c<16> + k_qr_band<16> (banded QR)" if 8 < n <= 16 else f"k_zgeev<double, {nb(n)}, 1, 0>")
+                 f"k_gehrd_c<{nb(n)}> + k_qr_band<{nb(n)}> (banded QR)" if n > 8 else f"k_zgeev<double, {nb(n)}, 1, 0>")
         rf = roofline(flops, nbytes, ms64, kname, f"zgeev_fp64_n{n}")
     rf["flops_per_matrix"] = nominal_flops(n)
 

@@ -1,5 +1,6 @@
-// Whole FP64 batches at orders 9..16 in two kernels, so that the QR
-// iteration runs on a banded tile and more matrices stay resident per CU.
+// FP64 batches (and lists of matrices from a batch) at orders 9..32 in two
+// kernels, so that the QR iteration runs on a banded tile and more matrices
+// stay resident per CU.
 //
 //   k_gehrd_c<NB>   the Hessenberg reduction alone (zgehrd, Toolbox.hpp:
 //                   1315-1469) on the dense tile, written in zgehrd's compact
@@ -15,7 +16,10 @@
 //
 // At NB = 16 the band holds 166 of the dense tile's 272 entries: 11.8 KB of
 // LDS per wavefront of four matrices instead of 18.6 KB, which with at most
-// 168 registers lets three wavefronts share a SIMD instead of two.  The
+// 168 registers lets three wavefronts share a SIMD instead of two; at
+// NB = 32, 19.5 KB per wavefront of two instead of 33.8 KB: two wavefronts
+// per SIMD instead of one (at 256 registers, part of the Schur-vector rows
+// spilled).  The
 // reduction round trip through HBM (the compact form written and read back)
 // costs 2 x 16 n^2 bytes per matrix.
 #include <hip/hip_runtime.h>
