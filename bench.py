@@ -496,7 +496,8 @@ def bench_varsize(a, cfg, veig, W, torch, dev, stream):
     u, c = torch.unique(dims.cpu(), return_counts=True)
     flops = float(sum(nominal_flops(int(n)) * int(k) for n, k in zip(u, c)))
     nbytes = int(sum(alg_bytes(int(n)) * int(k) for n, k in zip(u, c)))
-    rf = roofline(flops, nbytes, ms, "graph: bin kernels + k_zgeev_var<NB,P> per variant", "varsize_c3")
+    rf = roofline(flops, nbytes, ms, "graph: bin kernels + per-variant kernels (k_zgeev_var<8,P>, "
+                  "k_zgeev_var<16,2>, k_reduce_var<NB> + k_qr_var<NB> for NB = 16, 32)", "varsize_c3")
     rf["note"] = "whole graph timed (binning + three persistent bin kernels)"
     info = vb.info
     # backward error on a sample of every order

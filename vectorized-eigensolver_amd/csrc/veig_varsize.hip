@@ -19,13 +19,15 @@
 //   k_zgeev_var<NB,P> one persistent kernel per bin: a fixed grid walks the bin's
 //                  range, read from device memory, wavefront by wavefront; the
 //                  matrices of a wavefront then have (nearly) the same order.
+//                  Bins (16, 16) and (32, 32) run two such kernels instead,
+//                  k_reduce_var<NB> and k_qr_var<NB> (the banded path).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <string>
 #include <vector>
 
-#include "veig_common.hpp"
+#include "veig_band.hpp"
 
 using namespace veig;
 
@@ -161,6 +163,79 @@ __global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
   }
 }
 
+// Bins (16, 16) and (32, 32) on the two-kernel banded path (one index per
+// lane): k_reduce_var<NB> writes each matrix's compact Hessenberg form into
+// its V and E slots, then k_qr_var<NB> forms Q, iterates and solves on the
+// banded tile -- the LDS of a wavefront drops from 18.6 to 11.8 KB (NB = 16)
+// and from 33.8 to 19.5 KB (NB = 32), one more resident wavefront per SIMD.
+// Both walk the bin's range like k_zgeev_var.
+template <int NB>
+__device__ __forceinline__ int wave_max_order(int n) {
+  constexpr int LG = Group<double, NB, 1>::LG, MPW = Group<double, NB, 1>::MPW;
+  int nmax = 0;
+#pragma unroll
+  for (int q = 0; q < MPW; ++q) {
+    const int nq = __builtin_amdgcn_readlane(n, q * LG);
+    nmax = nq > nmax ? nq : nmax;
+  }
+  return nmax;
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_reduce_var(VarArgs a, int bin) {
+  using G = Group<double, NB, 1>;
+  __shared__ Tiles<double, NB, 1> sm;
+  constexpr int MPW = G::MPW;
+  const int64_t start = a.bins->start[bin], count = a.bins->count[bin];
+  const int64_t nwaves = (count + MPW - 1) / MPW;
+  for (int64_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
+    int lane = threadIdx.x;
+    asm volatile("" : "+v"(lane));
+    const int64_t idx = w * MPW + lane / G::LG;
+    const bool valid = idx < count;
+    const int64_t m = valid ? a.perm[start + idx] : 0;
+    const int n = valid ? a.dims[m] : 0;
+    const int nmax = wave_max_order<NB>(n);
+    G g = make_group<double, NB, 1>(sm, n, lane);
+    Geom Gm;
+    Gm.nmat = 0;
+    Gm.T = 1;
+    Gm.n = n;
+    const int64_t base = valid ? a.a_off[m] : 0, vbase = valid ? a.e_off[m] : 0;
+    band_reduce_one<NB>(g, Gm, base, vbase, nmax, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
+    wave_sync();
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(qr_band_waves<NB>()))) void k_qr_var(VarArgs a,
+                                                                                                       int bin) {
+  using G = Group<double, NB, 1, LAY_BAND>;
+  __shared__ BandTiles<NB> sm;
+  constexpr int MPW = G::MPW;
+  const int64_t start = a.bins->start[bin], count = a.bins->count[bin];
+  const int64_t nwaves = (count + MPW - 1) / MPW;
+  for (int64_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
+    int lane = threadIdx.x;
+    asm volatile("" : "+v"(lane));
+    const int64_t idx = w * MPW + lane / G::LG;
+    const bool valid = idx < count;
+    const int64_t m = valid ? a.perm[start + idx] : 0;
+    const int n = valid ? a.dims[m] : 0;
+    const int nmax = wave_max_order<NB>(n);
+    const G g = band_group<NB>(sm, n, lane);
+    Geom Gm;
+    Gm.nmat = 0;
+    Gm.T = 1;
+    Gm.n = n;
+    const int64_t base = valid ? a.a_off[m] : 0, vbase = valid ? a.e_off[m] : 0;
+    const int nfail =
+        band_qr_one<NB>(g, Gm, base, vbase, nmax, a.itmax, a.rV, a.iV, a.rE, a.iE, nullptr, nullptr);
+    if (a.info && valid && g.r == 0) a.info[m] = nfail;
+    wave_sync();
+  }
+}
+
 constexpr size_t kAlign = 256;
 size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -199,15 +274,21 @@ int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, con
   hipLaunchKernelGGL(k_bin_scatter, dim3(cgrid), dim3(256), 0, s, a);
   // persistent grids: a few resident waves per CU and bin (the bins run back
   // to back; every wave of a bin loops until the bin is drained)
-  const int defaults[kBins] = {8 * cus, 8 * cus, 8 * cus, 4 * cus, 4 * cus};
+  const int defaults[kBins] = {8 * cus, 8 * cus, 12 * cus, 4 * cus, 8 * cus};
   for (int b = 0; b < kBins; ++b) {
     const int g = (grid_per_bin && grid_per_bin[b] > 0) ? grid_per_bin[b] : defaults[b];
     switch (b) {
       case 0: hipLaunchKernelGGL((k_zgeev_var<8, 1>), dim3(g), dim3(64), 0, s, a, 0); break;
       case 1: hipLaunchKernelGGL((k_zgeev_var<8, 2>), dim3(g), dim3(64), 0, s, a, 1); break;
-      case 2: hipLaunchKernelGGL((k_zgeev_var<16, 1>), dim3(g), dim3(64), 0, s, a, 2); break;
+      case 2:
+        hipLaunchKernelGGL((k_reduce_var<16>), dim3(g), dim3(64), 0, s, a, 2);
+        hipLaunchKernelGGL((k_qr_var<16>), dim3(g), dim3(64), 0, s, a, 2);
+        break;
       case 3: hipLaunchKernelGGL((k_zgeev_var<16, 2>), dim3(g), dim3(64), 0, s, a, 3); break;
-      default: hipLaunchKernelGGL((k_zgeev_var<32, 1>), dim3(g), dim3(64), 0, s, a, 4); break;
+      default:
+        hipLaunchKernelGGL((k_reduce_var<32>), dim3(g), dim3(64), 0, s, a, 4);
+        hipLaunchKernelGGL((k_qr_var<32>), dim3(g), dim3(64), 0, s, a, 4);
+        break;
     }
   }
   VEIG_HIP(hipGetLastError());
