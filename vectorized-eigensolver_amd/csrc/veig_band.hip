@@ -52,16 +52,16 @@ __device__ __forceinline__ bool slot_matrix(const FusedArgs &a, int group, int64
 }
 
 template <int NB>
-__global__ __launch_bounds__(64) void k_gehrd_c(FusedArgs a) {
+__global__ __launch_bounds__(64 * reduce_waves<NB>()) void k_gehrd_c(FusedArgs a) {
   using G = Group<double, NB, 1>;
-  __shared__ Tiles<double, NB, 1> sm;
+  __shared__ ReduceTiles<NB> sm;
   int64_t m;
-  const bool valid = slot_matrix<G::MPW>(a, threadIdx.x / G::LG, m);
-  if (!__any(valid)) return;
+  // every wavefront of the workgroup holds the same lane groups
+  const bool valid = slot_matrix<G::MPW>(a, (threadIdx.x & 63) / G::LG, m);
+  if (!__syncthreads_or(valid)) return;  // workgroup-uniform
   const int n = valid ? a.G.n : 0;
-  G g = make_group<double, NB, 1>(sm, n, threadIdx.x);
   const int64_t base = valid ? a.G.mat_base(m) : 0, vbase = valid ? a.G.vec_base(m) : 0;
-  band_reduce_one<NB>(g, a.G, base, vbase, a.G.n, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
+  band_reduce_one<NB>(sm, n, a.G, base, vbase, a.G.n, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
 }
 
 template <int NB>
@@ -91,7 +91,7 @@ int launch_banded(const FusedArgs &a, hipStream_t s) {
   // a list launch gets the whole batch's grid: the list's length is only
   // known on the device, and it never exceeds the batch
   const unsigned grid = (unsigned)((a.G.nmat + mpw - 1) / mpw);
-  hipLaunchKernelGGL(k_gehrd_c<NB>, dim3(grid), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_gehrd_c<NB>, dim3(grid), dim3(64 * reduce_waves<NB>()), 0, s, a);
   hipLaunchKernelGGL(k_qr_band<NB>, dim3(grid), dim3(64), 0, s, a);
   VEIG_HIP(hipGetLastError());
   return VEIG_OK;

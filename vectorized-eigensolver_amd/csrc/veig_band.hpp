@@ -30,23 +30,189 @@ struct BandTiles {
 template <int NB>
 constexpr int qr_band_waves() { return NB == 16 ? 3 : 2; }
 
-// Reduction of the group's matrix (order g.n; nmax = the largest order in
-// the wavefront) into its compact form in the V and E planes.  g from
-// make_group (tile zeroed); every lane of the wavefront calls it.
+// Wavefronts of a workgroup that share one reduction, and the LDS it needs
+// beside the dense tiles: one partial product per wavefront and lane.  At
+// NB = 16 one wavefront (gehrd_compact) measured faster than two sharing
+// the work (7.6 vs 6.2 ms for 2^20 matrices): the tiles already allow two
+// wavefronts per SIMD there, against one at NB = 32 and half at NB = 64.
 template <int NB>
-__device__ __forceinline__ void band_reduce_one(const Group<double, NB, 1> &g, const Geom &G, int64_t base,
+constexpr int reduce_waves() { return NB == 64 ? 4 : (NB == 32 ? 2 : 1); }
+
+template <int NB>
+struct ReduceTiles {
+  Tiles<double, NB, 1> t;
+  double2 part[reduce_waves<NB>() > 1 ? reduce_waves<NB>() * 64 : 1];
+};
+
+// The Hessenberg reduction of gehrd_q (zgehrd with zlarfg's LAPACK sign,
+// Toolbox.hpp:1315-1469), leaving the compact form: v(i+2:n) below the
+// subdiagonal, tau(i) in tau_out of lane i.  The W = reduce_waves<NB>()
+// wavefronts of the workgroup hold the same lane groups and split each update
+// by chunks of CH = NB / W columns (right update) or rows (left update): a
+// wavefront takes its chunk of the lane's row or column and the reflector into
+// registers, puts its partial product in LDS, and after a barrier adds the W
+// partials in a fixed order and updates its chunk.  Every wavefront forms the
+// reflector from column i (the same arithmetic on the same data); wavefront 0
+// writes it.  Five workgroup barriers per reflector; the loop bound nmax must
+// be the same in every wavefront of the workgroup.
+template <int NB>
+__device__ __forceinline__ void gehrd_split(const Group<double, NB, 1> &g, int nmax, cx<double> &tau_out, int wq,
+                                            double2 *part) {
+  using R = double;
+  constexpr int LG = Group<R, NB, 1>::LG;
+  constexpr int W = reduce_waves<NB>();
+  constexpr int CH = NB / W;
+  const int n = g.n, r = g.r, lane = threadIdx.x & 63;
+  const bool in = r < n;
+  const int c0 = wq * CH;
+  for (int i = 0; i + 1 < nmax; ++i) {
+    // ---- reflector annihilating H(i+2:n, i) ----
+    const cx<R> hri = in ? ld<R>(g.at(r, i)) : mk<R>(0, 0);
+    const R xnorm2 = group_sum<LG, R>(r >= i + 2 && in ? abs2(hri) : R(0));
+    const bool live = i + 1 < n;
+    const cx<R> alpha = live ? ld<R>(g.at(i + 1, i)) : mk<R>(0, 0);
+    cx<R> tau = mk<R>(0, 0), scal = mk<R>(0, 0);
+    R beta = alpha.re;
+    if (live && !(xnorm2 == R(0) && alpha.im == R(0))) {
+      const R nrm = fsqrt(abs2(alpha) + xnorm2);
+      beta = alpha.re >= R(0) ? -nrm : nrm;
+      const R ib = frcp(beta);
+      tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
+      scal = inv(mk(alpha.re - beta, alpha.im));
+    }
+    cx<R> vr = mk<R>(0, 0);
+    if (live) {
+      if (r == i + 1) vr = mk<R>(1, 0);
+      else if (r >= i + 2 && in) vr = mul(hri, scal);
+    }
+    if (r == i) tau_out = tau;
+    __syncthreads();  // every wavefront has read column i
+    if (wq == 0) {
+      st<R>(g.vsh + r, vr);
+      if (live && in && r >= i + 1) st<R>(g.at(r, i), r == i + 1 ? mk<R>(beta, 0) : vr);
+    }
+    __syncthreads();
+    const bool mine = c0 + CH - 1 > i;  // this wavefront's chunk reaches past column i
+    // ---- H <- H (I - tau v v^H): row r, columns of the chunk ----
+    cx<R> h[CH], v[CH];
+    if (live && mine) {
+      cx<R> w = mk<R>(0, 0);
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        h[j] = ld<R>(g.at(r, c0 + j));
+        v[j] = ld<R>(g.vsh + c0 + j);  // zero at c <= i and beyond n
+      }
+#pragma unroll
+      for (int j = 0; j < CH; ++j)
+        if (c0 + j > i) w = add(w, mul(h[j], v[j]));
+      st<R>(part + wq * 64 + lane, w);
+    } else {
+      st<R>(part + wq * 64 + lane, mk<R>(0, 0));
+    }
+    __syncthreads();
+    if (live && mine) {
+      cx<R> w = mk<R>(0, 0);
+#pragma unroll
+      for (int q = 0; q < W; ++q) w = add(w, ld<R>(part + q * 64 + lane));
+      w = mul(tau, w);
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const int c = c0 + j;
+        if (c > i && c < n && in) st<R>(g.at(r, c), sub(h[j], mulc(w, v[j])));
+      }
+    }
+    __syncthreads();
+    // ---- H <- (I - conj(tau) v v^H) H: column r, rows of the chunk ----
+    if (live && mine) {
+      cx<R> y = mk<R>(0, 0);
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        h[j] = ld<R>(g.at(c0 + j, r));
+        v[j] = ld<R>(g.vsh + c0 + j);
+      }
+#pragma unroll
+      for (int j = 0; j < CH; ++j)
+        if (c0 + j > i) y = add(y, cmul(v[j], h[j]));
+      st<R>(part + wq * 64 + lane, y);
+    } else {
+      st<R>(part + wq * 64 + lane, mk<R>(0, 0));
+    }
+    __syncthreads();
+    if (live && mine) {
+      cx<R> y = mk<R>(0, 0);
+#pragma unroll
+      for (int q = 0; q < W; ++q) y = add(y, ld<R>(part + q * 64 + lane));
+      y = cmul(tau, y);  // conj(tau) * y
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const int k = c0 + j;
+        if (k > i && k < n && r > i && in) st<R>(g.at(k, r), sub(h[j], mul(v[j], y)));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Reduction of the lane group's matrix (order n; nmax = the largest order in
+// the workgroup) into its compact form in the V and E planes, by the W
+// wavefronts of the workgroup (wavefront index wq).  Every thread of the
+// workgroup calls it; the tile needs no zeroing beforehand.
+template <int NB>
+__device__ __forceinline__ void band_reduce_one(ReduceTiles<NB> &sm, int n, const Geom &G, int64_t base,
                                                 int64_t vbase, int nmax, const double *rA, const double *iA,
                                                 double *rV, double *iV, double *rE, double *iE) {
-  wave_sync();
-  load_rows<double, NB, 1>(g, G, base, rA, iA, NB);
-  wave_sync();
+  constexpr int CH = NB / reduce_waves<NB>();
+  const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
+  Group<double, NB, 1> g;
+  g.mi = lane / Group<double, NB, 1>::LG;
+  g.r = lane % Group<double, NB, 1>::LG;
+  g.n = n;
+  g.H = sm.t.H + g.mi * Tiles<double, NB, 1>::kTile;
+  g.vsh = sm.t.v + g.mi * Tiles<double, NB, 1>::kVec;
+  const int r = g.r, c0 = wq * CH;
   cx<double> tau = mk<double>(0, 0);
-  gehrd_compact<NB>(g, nmax, tau);
-  store_rows<double, NB, 1>(g, G, base, rV, iV);
-  if (g.r + 1 < g.n) {
-    rE[vbase + G.ve(g.r)] = tau.re;
-    iE[vbase + G.ve(g.r)] = tau.im;
+  if constexpr (reduce_waves<NB>() == 1) {
+    // one wavefront: runtime loops over the row (unrolled loads of the whole
+    // row would be hoisted together and cost registers)
+#pragma unroll
+    for (int c = 0; c < NB; ++c) st<double>(g.at(r, c), mk<double>(0, 0));
+    st<double>(g.vsh + r, mk<double>(0, 0));
+    wave_sync();
+    load_rows<double, NB, 1>(g, G, base, rA, iA, NB);
+    wave_sync();
+    gehrd_compact<NB>(g, nmax, tau);
+    store_rows<double, NB, 1>(g, G, base, rV, iV);
+  } else {
+    // row r, the wavefront's chunk of columns: the matrix, zeros outside it
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int c = c0 + j;
+      cx<double> x = mk<double>(0, 0);
+      if (r < n && c < n) {
+        const int64_t idx = base + G.el(r, c);
+        x = mk<double>(rA[idx], iA[idx]);
+      }
+      st<double>(g.at(r, c), x);
+    }
+    if (wq == 0) st<double>(g.vsh + r, mk<double>(0, 0));
+    __syncthreads();
+    gehrd_split<NB>(g, nmax, tau, wq, sm.part);
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int c = c0 + j;
+      if (r < n && c < n) {
+        const int64_t idx = base + G.el(r, c);
+        const cx<double> x = ld<double>(g.at(r, c));
+        rV[idx] = x.re;
+        iV[idx] = x.im;
+      }
+    }
   }
+  if (wq == 0 && r + 1 < n) {
+    rE[vbase + G.ve(r)] = tau.re;
+    iE[vbase + G.ve(r)] = tau.im;
+  }
+  __syncthreads();  // the tile is free for the workgroup's next matrices
 }
 
 // From the compact form: Q (zunghr, Toolbox.hpp:1470-1659), the QR iteration

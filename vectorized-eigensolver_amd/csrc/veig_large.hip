@@ -19,7 +19,7 @@
 #include <cstdint>
 #include <string>
 
-#include "veig_common.hpp"
+#include "veig_band.hpp"
 
 using namespace veig;
 
@@ -456,10 +456,11 @@ __global__ __launch_bounds__(128) void k_zgeev64_list(FusedArgs a) {
 // ---------------------------------------------------------------------------
 // Orders 33..64 in FP64, whole batches: two kernels.
 //
-// k_gehrd64 (one wavefront per matrix, the dense tile): the Hessenberg
-// reduction alone, in zgehrd's compact form -- H on and above the
-// subdiagonal, the reflector vectors v(i+2:n) below it, tau(i) -- written to
-// the V and E planes, which serve as its workspace.
+// k_gehrd64 (four wavefronts per matrix on its dense tile, each owning 16
+// columns / rows of every update): the Hessenberg reduction alone, in
+// zgehrd's compact form -- H on and above the subdiagonal, the reflector
+// vectors v(i+2:n) below it, tau(i) -- written to the V and E planes, which
+// serve as its workspace.
 //
 // k_qr64 (one workgroup of three wavefronts per matrix): the QR iteration and
 // the eigenvectors with only the band row <= col + 2 of H in LDS (35 KB
@@ -472,33 +473,12 @@ __global__ __launch_bounds__(128) void k_zgeev64_list(FusedArgs a) {
 // same swap.  About 40 KB of LDS and ~170 registers per wavefront let four
 // matrices share a CU, against two for k_zgeev64.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_gehrd64(FusedArgs a) {
-  __shared__ double2 H[kN * (kN + 1) + 1];
-  __shared__ double2 v[kN + 1];
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(64 * reduce_waves<kN>()) void k_gehrd64(FusedArgs a) {
+  __shared__ ReduceTiles<kN> sm;
   const int64_t m = xcd_block();
   if (m >= a.G.nmat) return;  // workgroup-uniform
-  const int n = a.G.n, nmax = a.G.n;
-  const int64_t base = a.G.mat_base(m), vbase = a.G.vec_base(m);
-  Group<double, kN, 1> g;
-  g.mi = 0;
-  g.r = lane;
-  g.n = n;
-  g.H = H;
-  g.vsh = v;
-#pragma unroll
-  for (int c = 0; c < kN; ++c) st<double>(g.at(lane, c), mk<double>(0, 0));
-  st<double>(g.vsh + lane, mk<double>(0, 0));
-  wave_sync();
-  load_rows<double, kN, 1>(g, a.G, base, a.rA, a.iA, kN);
-  wave_sync();
-  cx<double> tau = mk<double>(0, 0);
-  gehrd_compact<kN>(g, nmax, tau);
-  store_rows<double, kN, 1>(g, a.G, base, a.rV, a.iV);
-  if (lane + 1 < n) {
-    a.rE[vbase + a.G.ve(lane)] = tau.re;
-    a.iE[vbase + a.G.ve(lane)] = tau.im;
-  }
+  band_reduce_one<kN>(sm, a.G.n, a.G, a.G.mat_base(m), a.G.vec_base(m), a.G.n, a.rA, a.iA, a.rV, a.iV,
+                      a.rE, a.iE);
 }
 
 using GBand = Group<double, kN, 1, LAY_BAND>;
@@ -673,7 +653,7 @@ int launch_zgeev64(const FusedArgs &a, hipStream_t s) {
       return fail(VEIG_ERR_UNSUPPORTED, "orders above 32: list launches are FP64 only");
     }
   } else if constexpr (std::is_same<R, double>::value && OUT == OUT_V) {
-    hipLaunchKernelGGL(k_gehrd64, dim3((unsigned)a.G.nmat), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_gehrd64, dim3((unsigned)a.G.nmat), dim3(64 * reduce_waves<kN>()), 0, s, a);
     hipLaunchKernelGGL(k_qr64, dim3((unsigned)a.G.nmat), dim3(192), 0, s, a);
   } else {
     hipLaunchKernelGGL((k_zgeev64<R, OUT>), dim3((unsigned)a.G.nmat), dim3(128), 0, s, a);

@@ -182,28 +182,27 @@ __device__ __forceinline__ int wave_max_order(int n) {
 }
 
 template <int NB>
-__global__ __launch_bounds__(64) void k_reduce_var(VarArgs a, int bin) {
+__global__ __launch_bounds__(64 * reduce_waves<NB>()) void k_reduce_var(VarArgs a, int bin) {
   using G = Group<double, NB, 1>;
-  __shared__ Tiles<double, NB, 1> sm;
+  __shared__ ReduceTiles<NB> sm;
   constexpr int MPW = G::MPW;
   const int64_t start = a.bins->start[bin], count = a.bins->count[bin];
   const int64_t nwaves = (count + MPW - 1) / MPW;
+  // the workgroup's wavefronts share each wavefront-sized set of matrices
   for (int64_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
-    int lane = threadIdx.x;
+    int lane = threadIdx.x & 63;
     asm volatile("" : "+v"(lane));
     const int64_t idx = w * MPW + lane / G::LG;
     const bool valid = idx < count;
     const int64_t m = valid ? a.perm[start + idx] : 0;
     const int n = valid ? a.dims[m] : 0;
     const int nmax = wave_max_order<NB>(n);
-    G g = make_group<double, NB, 1>(sm, n, lane);
     Geom Gm;
     Gm.nmat = 0;
     Gm.T = 1;
     Gm.n = n;
     const int64_t base = valid ? a.a_off[m] : 0, vbase = valid ? a.e_off[m] : 0;
-    band_reduce_one<NB>(g, Gm, base, vbase, nmax, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
-    wave_sync();
+    band_reduce_one<NB>(sm, n, Gm, base, vbase, nmax, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
   }
 }
 
@@ -281,12 +280,12 @@ int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, con
       case 0: hipLaunchKernelGGL((k_zgeev_var<8, 1>), dim3(g), dim3(64), 0, s, a, 0); break;
       case 1: hipLaunchKernelGGL((k_zgeev_var<8, 2>), dim3(g), dim3(64), 0, s, a, 1); break;
       case 2:
-        hipLaunchKernelGGL((k_reduce_var<16>), dim3(g), dim3(64), 0, s, a, 2);
+        hipLaunchKernelGGL((k_reduce_var<16>), dim3(g), dim3(64 * reduce_waves<16>()), 0, s, a, 2);
         hipLaunchKernelGGL((k_qr_var<16>), dim3(g), dim3(64), 0, s, a, 2);
         break;
       case 3: hipLaunchKernelGGL((k_zgeev_var<16, 2>), dim3(g), dim3(64), 0, s, a, 3); break;
       default:
-        hipLaunchKernelGGL((k_reduce_var<32>), dim3(g), dim3(64), 0, s, a, 4);
+        hipLaunchKernelGGL((k_reduce_var<32>), dim3(g), dim3(64 * reduce_waves<32>()), 0, s, a, 4);
         hipLaunchKernelGGL((k_qr_var<32>), dim3(g), dim3(64), 0, s, a, 4);
         break;
     }
