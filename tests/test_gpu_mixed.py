@@ -101,12 +101,13 @@ def test_mixed_large_order_tiles(gpu):
         check(A, E, V, info, fb, be, 1e-13)
 
 
-@pytest.mark.parametrize("n", [16, 40])
-def test_mixed_all_fall_back(gpu, n):
+@pytest.mark.parametrize("n,tc", [(16, 1), (40, 1), (12, 4), (24, 4)])
+def test_mixed_all_fall_back(gpu, n, tc):
     # an unreachable tolerance sends every matrix to the FP64 solve of the
     # list (and its backward-error pass): FP64 results, flagged, info 0
+    # (orders 9..32: the banded kernels walking the list, tiles of tc)
     A = gaussian(64, n, 5600 + n)
-    E, V, info, fb, be = run_mixed(A, tol=1e-300)
+    E, V, info, fb, be = run_mixed(A, tc=tc, tol=1e-300)
     assert np.all(fb == 1) and np.all(info == 0)
     my_be = backward_error(A, E, V)
     assert my_be.max() < 64 * n * np.finfo(np.float64).eps
