@@ -63,6 +63,9 @@ const char *veig_version(void);
  * info (may be NULL): per matrix, the number of eigenvalues accepted after
  * `itmax` sweeps without a negligible subdiagonal (the reference caps at 30,
  * Toolbox.hpp:808, and reports nothing; itmax <= 0 selects 30).
+ * A, E and V are separate arrays: at orders 9..64 the call passes the
+ * Hessenberg reduction's compact form between its kernels through V and E,
+ * which then receive the results.
  * ------------------------------------------------------------------------- */
 int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
                double *rE, double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
