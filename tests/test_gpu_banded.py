@@ -1,6 +1,7 @@
 """GPU parity of the two-kernel FP64 paths (Hessenberg reduction in compact
 form, then the QR iteration and eigenvectors on a banded tile): orders 9..16
-(four matrices per wavefront) and 33..64 (one matrix per workgroup).
+(four matrices per wavefront), 17..24 (slots of 24 on 32 lanes), 25..32
+(two per wavefront) and 33..64 (one matrix per workgroup).
 
 Cases the split makes special: batches that leave lane groups or tiles
 partly empty (the reduction and the QR kernel must agree on which matrices
@@ -24,7 +25,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("n,N,T", [(9, 5, 1), (13, 7, 1), (16, 13, 1), (11, 12, 4), (16, 6, 2),
-                                   (10, 3, 3), (37, 3, 1), (64, 2, 2)])
+                                   (10, 3, 3), (17, 3, 1), (20, 5, 1), (24, 6, 2), (25, 5, 1),
+                                   (32, 3, 3), (37, 3, 1), (64, 2, 2)])
 def test_ragged_batches(gpu, n, N, T):
     A = H.gaussian(N, n, 4000 + 10 * n + N)
     Eo, Vo, _ = oracle_eig(A, 1)
@@ -32,7 +34,7 @@ def test_ragged_batches(gpu, n, N, T):
     check_parity(A, E, V, info, Eo, Vo)
 
 
-@pytest.mark.parametrize("n", [9, 12, 16, 36])
+@pytest.mark.parametrize("n", [9, 12, 16, 20, 28, 36])
 def test_reflectors_with_zero_tau(gpu, n):
     # upper Hessenberg input with some zero subdiagonal entries and a
     # triangular input: every or some reflectors of the reduction are the
@@ -53,7 +55,7 @@ def test_reflectors_with_zero_tau(gpu, n):
         assert np.max(H.backward_error(A, E, V)) < 64 * n * np.finfo(float).eps
 
 
-@pytest.mark.parametrize("n,N,T", [(9, 5, 1), (14, 8, 4), (33, 3, 1)])
+@pytest.mark.parametrize("n,N,T", [(9, 5, 1), (14, 8, 4), (22, 3, 1), (30, 4, 2), (33, 3, 1)])
 def test_schur_form_from_banded_tile(gpu, n, N, T):
     # VEIG_WRITE_SCHUR on the two-kernel path: T (from the banded tile) into
     # the input planes, exact zeros below the diagonal, diag(T) = E, and

@@ -51,32 +51,32 @@ __device__ __forceinline__ bool slot_matrix(const FusedArgs &a, int group, int64
   return m < a.G.nmat;
 }
 
-template <int NB>
+template <int NB, int LW>
 __global__ __launch_bounds__(64 * reduce_waves<NB>()) void k_gehrd_c(FusedArgs a) {
-  using G = Group<double, NB, 1>;
-  __shared__ ReduceTiles<NB> sm;
+  using G = Group<double, NB, 1, LAY_DENSE, LW>;
+  __shared__ ReduceTiles<NB, LW> sm;
   int64_t m;
   // every wavefront of the workgroup holds the same lane groups
   const bool valid = slot_matrix<G::MPW>(a, (threadIdx.x & 63) / G::LG, m);
   if (!__syncthreads_or(valid)) return;  // workgroup-uniform
   const int n = valid ? a.G.n : 0;
   const int64_t base = valid ? a.G.mat_base(m) : 0, vbase = valid ? a.G.vec_base(m) : 0;
-  band_reduce_one<NB>(sm, n, a.G, base, vbase, a.G.n, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
+  band_reduce_one<NB, LW>(sm, n, a.G, base, vbase, a.G.n, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
 }
 
-template <int NB>
+template <int NB, int LW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(qr_band_waves<NB>()))) void k_qr_band(FusedArgs a) {
-  using G = Group<double, NB, 1, LAY_BAND>;
-  __shared__ BandTiles<NB> sm;
+  using G = Group<double, NB, 1, LAY_BAND, LW>;
+  __shared__ BandTiles<NB, LW> sm;
   const int lane = threadIdx.x;
   int64_t m;
   const bool valid = slot_matrix<G::MPW>(a, lane / G::LG, m);
   if (!__any(valid)) return;
   const int n = valid ? a.G.n : 0;
-  const G g = band_group<NB>(sm, n, lane);
+  const G g = band_group<NB, LW>(sm, n, lane);
   const int64_t base = valid ? a.G.mat_base(m) : 0, vbase = valid ? a.G.vec_base(m) : 0;
   const bool schur = a.flags & VEIG_WRITE_SCHUR;
-  const int nfail = band_qr_one<NB>(g, a.G, base, vbase, a.G.n, a.itmax, a.rV, a.iV, a.rE, a.iE,
+  const int nfail = band_qr_one<NB, LW>(g, a.G, base, vbase, a.G.n, a.itmax, a.rV, a.iV, a.rE, a.iE,
                                     schur ? a.rA : nullptr, schur ? a.iA : nullptr);
   if (a.info && valid && g.r == 0) a.info[m] = nfail;
 }
@@ -85,18 +85,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(qr_band_wave
 
 namespace veig {
 
-template <int NB>
+template <int NB, int LW>
 int launch_banded(const FusedArgs &a, hipStream_t s) {
-  constexpr int mpw = Group<double, NB, 1>::MPW;
+  constexpr int mpw = Group<double, NB, 1, LAY_DENSE, LW>::MPW;
   // a list launch gets the whole batch's grid: the list's length is only
   // known on the device, and it never exceeds the batch
   const unsigned grid = (unsigned)((a.G.nmat + mpw - 1) / mpw);
-  hipLaunchKernelGGL(k_gehrd_c<NB>, dim3(grid), dim3(64 * reduce_waves<NB>()), 0, s, a);
-  hipLaunchKernelGGL(k_qr_band<NB>, dim3(grid), dim3(64), 0, s, a);
+  hipLaunchKernelGGL((k_gehrd_c<NB, LW>), dim3(grid), dim3(64 * reduce_waves<NB>()), 0, s, a);
+  hipLaunchKernelGGL((k_qr_band<NB, LW>), dim3(grid), dim3(64), 0, s, a);
   VEIG_HIP(hipGetLastError());
   return VEIG_OK;
 }
-template int launch_banded<16>(const FusedArgs &, hipStream_t);
-template int launch_banded<32>(const FusedArgs &, hipStream_t);
+template int launch_banded<16, 0>(const FusedArgs &, hipStream_t);
+template int launch_banded<24, 32>(const FusedArgs &, hipStream_t);
+template int launch_banded<32, 0>(const FusedArgs &, hipStream_t);
 
 }  // namespace veig

@@ -16,9 +16,9 @@ namespace veig {
 
 // LDS of a wavefront of the QR kernel: the band row <= col + 2 of each
 // group's tile and the reflector vector used while Q is formed
-template <int NB>
+template <int NB, int LW = 0>
 struct BandTiles {
-  using G = Group<double, NB, 1, LAY_BAND>;
+  using G = Group<double, NB, 1, LAY_BAND, LW>;
   static constexpr int kTile = G::kSize + 1;  // +1: each group starts on another bank
   static constexpr int kVec = NB + 1;
   double2 H[G::MPW * kTile];
@@ -26,7 +26,9 @@ struct BandTiles {
 };
 
 // wavefronts per SIMD the QR kernel's register allocation is held to: what
-// the LDS allows (11.8 KB per wavefront at NB = 16, 19.5 KB at NB = 32)
+// the LDS allows (11.8 KB per wavefront at NB = 16, 11.3 KB for two order-24
+// slots, 19.5 KB at NB = 32) where the registers can follow (three at
+// NB = 16; two otherwise, the Schur-vector rows being 24 or 32 entries)
 template <int NB>
 constexpr int qr_band_waves() { return NB == 16 ? 3 : 2; }
 
@@ -36,11 +38,11 @@ constexpr int qr_band_waves() { return NB == 16 ? 3 : 2; }
 // the work (7.6 vs 6.2 ms for 2^20 matrices): the tiles already allow two
 // wavefronts per SIMD there, against one at NB = 32 and half at NB = 64.
 template <int NB>
-constexpr int reduce_waves() { return NB == 64 ? 4 : (NB == 32 ? 2 : 1); }
+constexpr int reduce_waves() { return NB == 64 ? 4 : (NB > 16 ? 2 : 1); }
 
-template <int NB>
+template <int NB, int LW = 0>
 struct ReduceTiles {
-  Tiles<double, NB, 1> t;
+  Tiles<double, NB, 1, LW> t;
   double2 part[reduce_waves<NB>() > 1 ? reduce_waves<NB>() * 64 : 1];
 };
 
@@ -55,16 +57,17 @@ struct ReduceTiles {
 // reflector from column i (the same arithmetic on the same data); wavefront 0
 // writes it.  Five workgroup barriers per reflector; the loop bound nmax must
 // be the same in every wavefront of the workgroup.
-template <int NB>
-__device__ __forceinline__ void gehrd_split(const Group<double, NB, 1> &g, int nmax, cx<double> &tau_out, int wq,
-                                            double2 *part) {
+template <int NB, int LW = 0>
+__device__ __forceinline__ void gehrd_split(const Group<double, NB, 1, LAY_DENSE, LW> &g, int nmax,
+                                            cx<double> &tau_out, int wq, double2 *part) {
   using R = double;
-  constexpr int LG = Group<R, NB, 1>::LG;
+  constexpr int LG = Group<R, NB, 1, LAY_DENSE, LW>::LG;
   constexpr int W = reduce_waves<NB>();
   constexpr int CH = NB / W;
   const int n = g.n, r = g.r, lane = threadIdx.x & 63;
   const bool in = r < n;
   const int c0 = wq * CH;
+  const int rr = r < NB ? r : NB - 1;  // lanes past the slot's capacity read a valid entry and use none
   for (int i = 0; i + 1 < nmax; ++i) {
     // ---- reflector annihilating H(i+2:n, i) ----
     const cx<R> hri = in ? ld<R>(g.at(r, i)) : mk<R>(0, 0);
@@ -87,7 +90,7 @@ __device__ __forceinline__ void gehrd_split(const Group<double, NB, 1> &g, int n
     }
     if (r == i) tau_out = tau;
     __syncthreads();  // every wavefront has read column i
-    if (wq == 0) {
+    if (wq == 0 && r < NB) {
       st<R>(g.vsh + r, vr);
       if (live && in && r >= i + 1) st<R>(g.at(r, i), r == i + 1 ? mk<R>(beta, 0) : vr);
     }
@@ -99,7 +102,7 @@ __device__ __forceinline__ void gehrd_split(const Group<double, NB, 1> &g, int n
       cx<R> w = mk<R>(0, 0);
 #pragma unroll
       for (int j = 0; j < CH; ++j) {
-        h[j] = ld<R>(g.at(r, c0 + j));
+        h[j] = ld<R>(g.at(rr, c0 + j));
         v[j] = ld<R>(g.vsh + c0 + j);  // zero at c <= i and beyond n
       }
 #pragma unroll
@@ -127,7 +130,7 @@ __device__ __forceinline__ void gehrd_split(const Group<double, NB, 1> &g, int n
       cx<R> y = mk<R>(0, 0);
 #pragma unroll
       for (int j = 0; j < CH; ++j) {
-        h[j] = ld<R>(g.at(c0 + j, r));
+        h[j] = ld<R>(g.at(c0 + j, rr));
         v[j] = ld<R>(g.vsh + c0 + j);
       }
 #pragma unroll
@@ -157,18 +160,19 @@ __device__ __forceinline__ void gehrd_split(const Group<double, NB, 1> &g, int n
 // the workgroup) into its compact form in the V and E planes, by the W
 // wavefronts of the workgroup (wavefront index wq).  Every thread of the
 // workgroup calls it; the tile needs no zeroing beforehand.
-template <int NB>
-__device__ __forceinline__ void band_reduce_one(ReduceTiles<NB> &sm, int n, const Geom &G, int64_t base,
+template <int NB, int LW = 0>
+__device__ __forceinline__ void band_reduce_one(ReduceTiles<NB, LW> &sm, int n, const Geom &G, int64_t base,
                                                 int64_t vbase, int nmax, const double *rA, const double *iA,
                                                 double *rV, double *iV, double *rE, double *iE) {
+  using Gd = Group<double, NB, 1, LAY_DENSE, LW>;
   constexpr int CH = NB / reduce_waves<NB>();
   const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
-  Group<double, NB, 1> g;
-  g.mi = lane / Group<double, NB, 1>::LG;
-  g.r = lane % Group<double, NB, 1>::LG;
+  Gd g;
+  g.mi = lane / Gd::LG;
+  g.r = lane % Gd::LG;
   g.n = n;
-  g.H = sm.t.H + g.mi * Tiles<double, NB, 1>::kTile;
-  g.vsh = sm.t.v + g.mi * Tiles<double, NB, 1>::kVec;
+  g.H = sm.t.H + g.mi * Tiles<double, NB, 1, LW>::kTile;
+  g.vsh = sm.t.v + g.mi * Tiles<double, NB, 1, LW>::kVec;
   const int r = g.r, c0 = wq * CH;
   cx<double> tau = mk<double>(0, 0);
   if constexpr (reduce_waves<NB>() == 1) {
@@ -184,17 +188,19 @@ __device__ __forceinline__ void band_reduce_one(ReduceTiles<NB> &sm, int n, cons
     store_rows<double, NB, 1>(g, G, base, rV, iV);
   } else {
     // row r, the wavefront's chunk of columns: the matrix, zeros outside it
+    if (r < NB) {
 #pragma unroll
-    for (int j = 0; j < CH; ++j) {
-      const int c = c0 + j;
-      cx<double> x = mk<double>(0, 0);
-      if (r < n && c < n) {
-        const int64_t idx = base + G.el(r, c);
-        x = mk<double>(rA[idx], iA[idx]);
+      for (int j = 0; j < CH; ++j) {
+        const int c = c0 + j;
+        cx<double> x = mk<double>(0, 0);
+        if (r < n && c < n) {
+          const int64_t idx = base + G.el(r, c);
+          x = mk<double>(rA[idx], iA[idx]);
+        }
+        st<double>(g.at(r, c), x);
       }
-      st<double>(g.at(r, c), x);
+      if (wq == 0) st<double>(g.vsh + r, mk<double>(0, 0));
     }
-    if (wq == 0) st<double>(g.vsh + r, mk<double>(0, 0));
     __syncthreads();
     gehrd_split<NB>(g, nmax, tau, wq, sm.part);
 #pragma unroll
@@ -221,11 +227,11 @@ __device__ __forceinline__ void band_reduce_one(ReduceTiles<NB> &sm, int n, cons
 // overwrite the compact form; T to rT/iT if given.  Returns the count of
 // eigenvalues accepted at the iteration cap.  g: the group's banded tile and
 // vector (g.H, g.vsh, g.r, g.n set; the tile is zeroed here).
-template <int NB>
-__device__ __forceinline__ int band_qr_one(const Group<double, NB, 1, LAY_BAND> &g, const Geom &G, int64_t base,
+template <int NB, int LW = 0>
+__device__ __forceinline__ int band_qr_one(const Group<double, NB, 1, LAY_BAND, LW> &g, const Geom &G, int64_t base,
                                            int64_t vbase, int nmax, int itmax, double *rV, double *iV,
                                            double *rE, double *iE, double *rT, double *iT) {
-  using Gb = Group<double, NB, 1, LAY_BAND>;
+  using Gb = Group<double, NB, 1, LAY_BAND, LW>;
   const int r = g.r, n = g.n;
   for (int i = r; i < Gb::kSize; i += Gb::LG) st<double>(g.H + i, mk<double>(0, 0));
   wave_sync();
@@ -250,7 +256,7 @@ __device__ __forceinline__ int band_qr_one(const Group<double, NB, 1, LAY_BAND> 
       }
       tau = mk<double>(rE[vbase + G.ve(i)], iE[vbase + G.ve(i)]);
     }
-    st<double>(g.vsh + r, vl);
+    if (r < NB) st<double>(g.vsh + r, vl);
     wave_sync();
     cx<double> w = mk<double>(0, 0);
 #pragma unroll
@@ -301,15 +307,15 @@ __device__ __forceinline__ int band_qr_one(const Group<double, NB, 1, LAY_BAND> 
 }
 
 // the banded group of lane `lane` in a wavefront's BandTiles
-template <int NB>
-__device__ __forceinline__ Group<double, NB, 1, LAY_BAND> band_group(BandTiles<NB> &sm, int n, int lane) {
-  using Gb = Group<double, NB, 1, LAY_BAND>;
+template <int NB, int LW = 0>
+__device__ __forceinline__ Group<double, NB, 1, LAY_BAND, LW> band_group(BandTiles<NB, LW> &sm, int n, int lane) {
+  using Gb = Group<double, NB, 1, LAY_BAND, LW>;
   Gb g;
   g.mi = lane / Gb::LG;
   g.r = lane % Gb::LG;
   g.n = n;
-  g.H = sm.H + g.mi * BandTiles<NB>::kTile;
-  g.vsh = sm.v + g.mi * BandTiles<NB>::kVec;
+  g.H = sm.H + g.mi * BandTiles<NB, LW>::kTile;
+  g.vsh = sm.v + g.mi * BandTiles<NB, LW>::kVec;
   return g;
 }
 

@@ -78,10 +78,10 @@ __device__ __forceinline__ int64_t xcd_block() { return xcd_remap(blockIdx.x, gr
 // --------------------------------------------------------------------------
 // common kernel prologue: lane group setup and LDS tile
 // --------------------------------------------------------------------------
-template <typename R, int NB, int P = 1>
+template <typename R, int NB, int P = 1, int LW = 0>
 struct Tiles {
   using L2 = typename lds2<R>::type;
-  static constexpr int MPW = Group<R, NB, P>::MPW;
+  static constexpr int MPW = Group<R, NB, P, LAY_DENSE, LW>::MPW;
   static constexpr int LD = NB + 1;
   // one element of padding between the groups' tiles (and vectors) starts
   // each group on a different LDS bank: the groups' broadcast reads of their
@@ -415,9 +415,10 @@ inline void launch_one(const FusedArgs &a, hipStream_t s) {
 template <typename R, int OUT>
 int launch_zgeev64(const FusedArgs &a, hipStream_t s);
 
-// FP64 batches and lists at NB / 2 < n <= NB: Hessenberg reduction, then
-// the QR iteration and eigenvectors on a banded tile (two kernels)
-template <int NB>
+// FP64 batches and lists of orders up to NB on lane groups of LW lanes
+// (0: NB): Hessenberg reduction, then the QR iteration and eigenvectors on
+// a banded tile (two kernels)
+template <int NB, int LW = 0>
 int launch_banded(const FusedArgs &a, hipStream_t s);
 
 template <typename R, int OUT = OUT_V>
@@ -446,7 +447,7 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
       break;
     case 32:
       if constexpr (!f32 && OUT == OUT_V) {
-        if (p == 1 && !a.mask) return launch_banded<32>(a, s);
+        if (p == 1 && !a.mask) return a.G.n <= 24 ? launch_banded<24, 32>(a, s) : launch_banded<32>(a, s);
       }
       if constexpr (f32) {
         if (p == 2) {

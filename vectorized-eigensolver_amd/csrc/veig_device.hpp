@@ -194,10 +194,12 @@ __device__ __forceinline__ void uniform_dispatch(int k, F&& f) {
 // eigenvalues-only QR iteration, which never leaves that band.
 enum { LAY_DENSE = 0, LAY_BAND = 1 };
 
-template <typename R, int NB, int P = 1, int LAY = LAY_DENSE>
+// LW: lanes per matrix when wider than NB / P (a slot of capacity NB on a
+// lane group of LW lanes, the lanes past NB / P idle in row operations)
+template <typename R, int NB, int P = 1, int LAY = LAY_DENSE, int LW = 0>
 struct Group {
-  static_assert(NB % P == 0 && 64 % (NB / P) == 0, "lane group must divide the wavefront");
-  static constexpr int LG = NB / P;    // lanes per matrix
+  static constexpr int LG = LW ? LW : NB / P;  // lanes per matrix
+  static_assert(NB <= LG * P && 64 % LG == 0, "lane group must divide the wavefront and hold NB indices");
   static constexpr int LD = NB + 1;
   static constexpr int MPW = 64 / LG;  // matrices per wavefront
   // complex entries of one matrix's tile
@@ -345,10 +347,11 @@ __device__ __forceinline__ void gehrd_q(const Group<R, NB, P>& g, cx<R> (&z)[P][
 // waits on its own LDS round trip; entries at or left of column i are
 // loaded and left alone, chunks wholly there are skipped.  With one chunk
 // (NB <= 16) the update reuses the entries the product loaded.
-template <int NB>
-__device__ __forceinline__ void gehrd_compact(const Group<double, NB, 1> &g, int nmax, cx<double> &tau_out) {
+template <int NB, int LW = 0>
+__device__ __forceinline__ void gehrd_compact(const Group<double, NB, 1, LAY_DENSE, LW> &g, int nmax,
+                                              cx<double> &tau_out) {
   using R = double;
-  constexpr int LG = Group<R, NB, 1>::LG;
+  constexpr int LG = Group<R, NB, 1, LAY_DENSE, LW>::LG;
   constexpr int CH = NB < 16 ? NB : 16;  // entries per chunk
   constexpr int NCH = NB / CH;
   const int n = g.n, r = g.r;
@@ -373,7 +376,7 @@ __device__ __forceinline__ void gehrd_compact(const Group<double, NB, 1> &g, int
       if (r == i + 1) vr = mk<R>(1, 0);
       else if (r >= i + 2 && in) vr = mul(hri, scal);
     }
-    st<R>(g.vsh + r, vr);
+    if (r < NB) st<R>(g.vsh + r, vr);
     if (live && in && r >= i + 1) st<R>(g.at(r, i), r == i + 1 ? mk<R>(beta, 0) : vr);
     if (r == i) tau_out = tau;
     wave_sync();
@@ -486,11 +489,11 @@ __device__ __forceinline__ void larfg2(cx<R> a0, cx<R> a1, cx<R>& t1, cx<R>& v2,
 // TZ = false is the eigenvalues-only sweep (single_shift_qr, Toolbox.hpp:
 // 372-522): H is updated inside the active block L..I only (columns <= I,
 // rows >= L) and there are no Schur vectors.
-template <int KC, bool TZ = true, typename R, int NB, int P, int LAY>
-__device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY>& g, cx<R> (&z)[P][NB], bool act,
+template <int KC, bool TZ = true, typename R, int NB, int P, int LAY, int LW>
+__device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], bool act,
                                            int L, int I, cx<R> a0, cx<R> a1, int Krt = 0) {
-  using L2 = typename Group<R, NB, P, LAY>::L2;
-  constexpr int LG = Group<R, NB, P, LAY>::LG;
+  using L2 = typename Group<R, NB, P, LAY, LW>::L2;
+  constexpr int LG = Group<R, NB, P, LAY, LW>::LG;
   const int n = g.n;
   const int K = KC >= 0 ? KC : Krt;
   cx<R> b0 = a0, b1 = a1;
@@ -562,8 +565,8 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY>& g, cx<R> 
   wave_sync();
 }
 
-template <int K, int KEND, bool TZ = true, typename R, int NB, int P, int LAY>
-__device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY>& g, cx<R> (&z)[P][NB], bool ready,
+template <int K, int KEND, bool TZ = true, typename R, int NB, int P, int LAY, int LW>
+__device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], bool ready,
                                             int L, int I, cx<R> a0, cx<R> a1, int kmin, int kmax) {
   if constexpr (K < KEND) {
     if (K >= kmin && K < kmax) sweep_step<K, TZ>(g, z, ready && K >= L && K < I, L, I, a0, a1);
@@ -586,11 +589,11 @@ __device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY>& g, cx<R>
 // no Schur vectors and H updated inside the active block only; the active
 // block evolves exactly as with TZ = true, so the eigenvalues are the same.
 // --------------------------------------------------------------------------
-template <typename R, int NB, int P, bool TZ = true, int LAY = LAY_DENSE>
-__device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY>& g, cx<R> (&z)[P][NB], int itmax,
+template <typename R, int NB, int P, bool TZ = true, int LAY = LAY_DENSE, int LW = 0>
+__device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], int itmax,
                                      bool make_real_subdiag) {
-  constexpr int LG = Group<R, NB, P, LAY>::LG;
-  constexpr int MPW = Group<R, NB, P, LAY>::MPW;
+  constexpr int LG = Group<R, NB, P, LAY, LW>::LG;
+  constexpr int MPW = Group<R, NB, P, LAY, LW>::MPW;
   const int n = g.n;
   const R ulp = limits<R>::eps;
   const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
@@ -943,8 +946,8 @@ __device__ __forceinline__ void backtransform(const Group<R, NB, P>& g, cx<R> (&
 // (rows < k).  At step j lanes k > j read column j of T, which lane j only
 // starts to overwrite at step j-1, so T and the solutions share the tile.
 // --------------------------------------------------------------------------
-template <typename R, int NB, int LAY = LAY_DENSE>
-__device__ __forceinline__ cx<R> trevc_solve_lds(const Group<R, NB, 1, LAY>& g, int nmax) {
+template <typename R, int NB, int LAY = LAY_DENSE, int LW = 0>
+__device__ __forceinline__ cx<R> trevc_solve_lds(const Group<R, NB, 1, LAY, LW>& g, int nmax) {
   const int r = g.r, n = g.n;
   const R ulp = limits<R>::eps;
   const R smlnum = limits<R>::tiny * (R(n > 0 ? n : 1) / ulp);
