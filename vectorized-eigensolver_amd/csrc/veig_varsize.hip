@@ -20,7 +20,8 @@
 //                  range, read from device memory, wavefront by wavefront; the
 //                  matrices of a wavefront then have (nearly) the same order.
 //                  Bins (16, 16) and (32, 32) run two such kernels instead,
-//                  k_reduce_var<NB> and k_qr_var<NB> (the banded path).
+//                  k_reduce_var and k_qr_var (the banded path); the (32, 32)
+//                  bin's orders <= 24 in slots of capacity 24.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -34,6 +35,8 @@ using namespace veig;
 namespace {
 
 constexpr int kMaxN = 32;
+// orders of the width-32 bin that run in slots of capacity 24
+constexpr int kSmall = 24;
 // kernel variants (bins): lane-group width NB and slots P, i.e. NB/P lanes
 // per matrix -- the threads-per-matrix choice of the autotuner
 constexpr int kBins = 5;
@@ -46,6 +49,7 @@ struct BinState {
   int32_t cursor[kMaxN + 1];   // scatter cursor per order
   int64_t start[kBins];        // bin range in the permutation
   int64_t count[kBins];
+  int64_t small[kBins];        // leading part of a width-32 bin with orders <= 24
   int32_t var_of_n[kMaxN + 1]; // kernel variant per order
 };
 
@@ -74,6 +78,7 @@ __global__ void k_bin_init(BinState *b, const int8_t *var_table) {
   if (i < kBins) {
     b->start[i] = 0;
     b->count[i] = 0;
+    b->small[i] = 0;
   }
 }
 
@@ -103,6 +108,7 @@ __global__ void k_bin_scan(BinState *b) {
       if (b->var_of_n[n] != bi) continue;
       b->cursor[n] = (int32_t)pos;
       pos += b->hist[n];
+      if (n <= kSmall) b->small[bi] = pos - b->start[bi];
     }
     b->count[bi] = pos - b->start[bi];
   }
@@ -169,9 +175,21 @@ __global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
 // banded tile -- the LDS of a wavefront drops from 18.6 to 11.8 KB (NB = 16)
 // and from 33.8 to 19.5 KB (NB = 32), one more resident wavefront per SIMD.
 // Both walk the bin's range like k_zgeev_var.
-template <int NB>
+// the range of a bin, or of its part 0 (orders <= kSmall) or part 1 (the rest)
+__device__ __forceinline__ void bin_part(const VarArgs &a, int bin, int part, int64_t &start, int64_t &count) {
+  start = a.bins->start[bin];
+  count = a.bins->count[bin];
+  const int64_t small = a.bins->small[bin];
+  if (part == 0) {
+    count = small;
+  } else if (part == 1) {
+    start += small;
+    count -= small;
+  }
+}
+
+template <int LG, int MPW>
 __device__ __forceinline__ int wave_max_order(int n) {
-  constexpr int LG = Group<double, NB, 1>::LG, MPW = Group<double, NB, 1>::MPW;
   int nmax = 0;
 #pragma unroll
   for (int q = 0; q < MPW; ++q) {
@@ -181,12 +199,14 @@ __device__ __forceinline__ int wave_max_order(int n) {
   return nmax;
 }
 
-template <int NB>
-__global__ __launch_bounds__(64 * reduce_waves<NB>()) void k_reduce_var(VarArgs a, int bin) {
-  using G = Group<double, NB, 1>;
-  __shared__ ReduceTiles<NB> sm;
+template <int NB, int LW>
+__global__ __launch_bounds__(64 * reduce_waves<NB>()) __attribute__((amdgpu_waves_per_eu(2))) void k_reduce_var(
+    VarArgs a, int bin, int part) {
+  using G = Group<double, NB, 1, LAY_DENSE, LW>;
+  __shared__ ReduceTiles<NB, LW> sm;
   constexpr int MPW = G::MPW;
-  const int64_t start = a.bins->start[bin], count = a.bins->count[bin];
+  int64_t start, count;
+  bin_part(a, bin, part, start, count);
   const int64_t nwaves = (count + MPW - 1) / MPW;
   // the workgroup's wavefronts share each wavefront-sized set of matrices
   for (int64_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
@@ -196,23 +216,25 @@ __global__ __launch_bounds__(64 * reduce_waves<NB>()) void k_reduce_var(VarArgs 
     const bool valid = idx < count;
     const int64_t m = valid ? a.perm[start + idx] : 0;
     const int n = valid ? a.dims[m] : 0;
-    const int nmax = wave_max_order<NB>(n);
+    const int nmax = wave_max_order<G::LG, MPW>(n);
     Geom Gm;
     Gm.nmat = 0;
     Gm.T = 1;
     Gm.n = n;
     const int64_t base = valid ? a.a_off[m] : 0, vbase = valid ? a.e_off[m] : 0;
-    band_reduce_one<NB>(sm, n, Gm, base, vbase, nmax, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
+    band_reduce_one<NB, LW>(sm, n, Gm, base, vbase, nmax, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE);
   }
 }
 
-template <int NB>
+template <int NB, int LW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(qr_band_waves<NB>()))) void k_qr_var(VarArgs a,
-                                                                                                       int bin) {
-  using G = Group<double, NB, 1, LAY_BAND>;
-  __shared__ BandTiles<NB> sm;
+                                                                                                       int bin,
+                                                                                                       int part) {
+  using G = Group<double, NB, 1, LAY_BAND, LW>;
+  __shared__ BandTiles<NB, LW> sm;
   constexpr int MPW = G::MPW;
-  const int64_t start = a.bins->start[bin], count = a.bins->count[bin];
+  int64_t start, count;
+  bin_part(a, bin, part, start, count);
   const int64_t nwaves = (count + MPW - 1) / MPW;
   for (int64_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
     int lane = threadIdx.x;
@@ -221,15 +243,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(qr_band_wave
     const bool valid = idx < count;
     const int64_t m = valid ? a.perm[start + idx] : 0;
     const int n = valid ? a.dims[m] : 0;
-    const int nmax = wave_max_order<NB>(n);
-    const G g = band_group<NB>(sm, n, lane);
+    const int nmax = wave_max_order<G::LG, MPW>(n);
+    const G g = band_group<NB, LW>(sm, n, lane);
     Geom Gm;
     Gm.nmat = 0;
     Gm.T = 1;
     Gm.n = n;
     const int64_t base = valid ? a.a_off[m] : 0, vbase = valid ? a.e_off[m] : 0;
     const int nfail =
-        band_qr_one<NB>(g, Gm, base, vbase, nmax, a.itmax, a.rV, a.iV, a.rE, a.iE, nullptr, nullptr);
+        band_qr_one<NB, LW>(g, Gm, base, vbase, nmax, a.itmax, a.rV, a.iV, a.rE, a.iE, nullptr, nullptr);
     if (a.info && valid && g.r == 0) a.info[m] = nfail;
     wave_sync();
   }
@@ -280,13 +302,16 @@ int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, con
       case 0: hipLaunchKernelGGL((k_zgeev_var<8, 1>), dim3(g), dim3(64), 0, s, a, 0); break;
       case 1: hipLaunchKernelGGL((k_zgeev_var<8, 2>), dim3(g), dim3(64), 0, s, a, 1); break;
       case 2:
-        hipLaunchKernelGGL((k_reduce_var<16>), dim3(g), dim3(64 * reduce_waves<16>()), 0, s, a, 2);
-        hipLaunchKernelGGL((k_qr_var<16>), dim3(g), dim3(64), 0, s, a, 2);
+        hipLaunchKernelGGL((k_reduce_var<16, 0>), dim3(g), dim3(64 * reduce_waves<16>()), 0, s, a, 2, -1);
+        hipLaunchKernelGGL((k_qr_var<16, 0>), dim3(g), dim3(64), 0, s, a, 2, -1);
         break;
       case 3: hipLaunchKernelGGL((k_zgeev_var<16, 2>), dim3(g), dim3(64), 0, s, a, 3); break;
       default:
-        hipLaunchKernelGGL((k_reduce_var<32>), dim3(g), dim3(64 * reduce_waves<32>()), 0, s, a, 4);
-        hipLaunchKernelGGL((k_qr_var<32>), dim3(g), dim3(64), 0, s, a, 4);
+        // orders <= 24 of the bin in slots of 24, the rest in slots of 32
+        hipLaunchKernelGGL((k_reduce_var<kSmall, 32>), dim3(g), dim3(64 * reduce_waves<kSmall>()), 0, s, a, 4, 0);
+        hipLaunchKernelGGL((k_qr_var<kSmall, 32>), dim3(g), dim3(64), 0, s, a, 4, 0);
+        hipLaunchKernelGGL((k_reduce_var<32, 0>), dim3(g), dim3(64 * reduce_waves<32>()), 0, s, a, 4, 1);
+        hipLaunchKernelGGL((k_qr_var<32, 0>), dim3(g), dim3(64), 0, s, a, 4, 1);
         break;
     }
   }
