@@ -1,7 +1,7 @@
 """GPU parity of the two-kernel FP64 paths (Hessenberg reduction in compact
 form, then the QR iteration and eigenvectors on a banded tile): orders 9..16
-(four matrices per wavefront), 17..24 (slots of 24 on 32 lanes), 25..32
-(two per wavefront) and 33..64 (one matrix per workgroup).
+(four matrices per wavefront), 17..24 and 25..28 (slots of 24 and 28 on 32
+lanes), 29..32 (two per wavefront) and 33..64 (one matrix per workgroup).
 
 Cases the split makes special: batches that leave lane groups or tiles
 partly empty (the reduction and the QR kernel must agree on which matrices
@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("n,N,T", [(9, 5, 1), (13, 7, 1), (16, 13, 1), (11, 12, 4), (16, 6, 2),
                                    (10, 3, 3), (17, 3, 1), (20, 5, 1), (24, 6, 2), (25, 5, 1),
-                                   (32, 3, 3), (37, 3, 1), (64, 2, 2)])
+                                   (28, 4, 2), (29, 3, 1), (32, 3, 3), (37, 3, 1), (64, 2, 2)])
 def test_ragged_batches(gpu, n, N, T):
     A = H.gaussian(N, n, 4000 + 10 * n + N)
     Eo, Vo, _ = oracle_eig(A, 1)

@@ -98,6 +98,7 @@ int launch_banded(const FusedArgs &a, hipStream_t s) {
 }
 template int launch_banded<16, 0>(const FusedArgs &, hipStream_t);
 template int launch_banded<24, 32>(const FusedArgs &, hipStream_t);
+template int launch_banded<28, 32>(const FusedArgs &, hipStream_t);
 template int launch_banded<32, 0>(const FusedArgs &, hipStream_t);
 
 }  // namespace veig

@@ -447,7 +447,10 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
       break;
     case 32:
       if constexpr (!f32 && OUT == OUT_V) {
-        if (p == 1 && !a.mask) return a.G.n <= 24 ? launch_banded<24, 32>(a, s) : launch_banded<32>(a, s);
+        // slots of capacity 24 or 28 where they fit: shorter Schur-vector rows, fewer spills
+        if (p == 1 && !a.mask)
+          return a.G.n <= 24 ? launch_banded<24, 32>(a, s)
+                             : (a.G.n <= 28 ? launch_banded<28, 32>(a, s) : launch_banded<32>(a, s));
       }
       if constexpr (f32) {
         if (p == 2) {
