@@ -17,6 +17,8 @@
 //   call_zgemv / call_backsubstitution           utils.cpp:218 / 269
 //   zlahqr_ (eigenvalues only)                   Toolbox.hpp:1663
 //   zlahqr (polynomial roots kernel)             Toolbox.hpp:1894
+#include <omp.h>
+
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -77,11 +79,44 @@ void clear_below_subdiag(int dim, int block_cnt, int thread_cnt, double *rA, dou
 
 extern "C" {
 
+// Threads of the reference's OpenMP loops for calls from this thread
+// (<= 0: the number of processors); returns the previous setting.  The
+// bit-exact comparisons run the reference single-threaded so that they
+// compare arithmetic, not the OpenMP schedule.
+int ref_set_threads(int nthreads) {
+  const int prev = omp_get_max_threads();
+  omp_set_num_threads(nthreads > 0 ? nthreads : omp_get_num_procs());
+  return prev;
+}
+
 // The reference's fused host call, exactly as shipped.
 void ref_eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rE,
                      double *iE, double *rV, double *iV) {
   MuteCout mute;
   eigensolver(dim, block_cnt, thread_cnt, rA, iA, rE, iE, rV, iV);
+}
+
+// The stage sequence of eigensolver() exactly as shipped (utils.cpp:598-615,
+// the reflector entries left below the subdiagonal during QR), but with tau
+// and the work vectors zeroed: the shipped call takes them from
+// posix_memalign uninitialised, and the reference's zgemv forms beta*y
+// before adding A*x even for beta = 0 (Toolbox.hpp:1070-1080), so its result
+// depends on whatever the heap held (NaN bytes give NaN eigenvalues, a
+// negative value a -0 that can flip a later sign test).  Zeroed buffers make
+// the as-shipped path deterministic, as the oracle assumes.
+void ref_eigensolver_shipped(int dim, int block_cnt, int thread_cnt, double *rA, double *iA,
+                             double *rE, double *iE, double *rV, double *iV) {
+  const size_t nm = (size_t)block_cnt * thread_cnt;
+  const size_t sizem = (size_t)dim * dim * nm, sizev = (size_t)dim * nm;
+  double *rtau = alloc(sizev), *itau = alloc(sizev);
+  double *rwork = alloc(2 * sizev), *iwork = alloc(2 * sizev);
+  test_zgehrd(rA, iA, rtau, itau, rwork, iwork, dim, block_cnt, thread_cnt);
+  std::memcpy(rV, rA, sizem * sizeof(double));
+  std::memcpy(iV, iA, sizem * sizeof(double));
+  test_zunghr(rV, iV, rtau, itau, rwork, iwork, dim, block_cnt, thread_cnt);
+  zlahqr_new(rE, iE, rA, iA, rV, iV, dim, dim * dim, block_cnt, thread_cnt);
+  test_ztrevc(rA, iA, rV, iV, rwork, iwork, dim, dim * dim, block_cnt, thread_cnt);
+  free(rtau); free(itau); free(rwork); free(iwork);
 }
 
 // The same four reference stages in the same order and with the same work

@@ -108,13 +108,17 @@ def load_ref(exact: bool = True):
         return None
     lib = ctypes.CDLL(str(path))
     i, d = ctypes.c_int, ctypes.c_double
-    for name in ("ref_eigensolver", "ref_eigensolver_clean"):
-        getattr(lib, name).argtypes = [i, i, i] + [_P] * 6
+    for name in ("ref_eigensolver", "ref_eigensolver_clean", "ref_eigensolver_shipped"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [i, i, i] + [_P] * 6
     for name in ("ref_zgehrd", "ref_zunghr", "ref_hessenberg", "ref_ztrevc", "ref_backsubstitution"):
         getattr(lib, name).argtypes = [i, i, i] + [_P] * 4
     lib.ref_zlahqr_new.argtypes = [i, i, i] + [_P] * 6
     lib.ref_call_zlahqr_new.argtypes = [i, i, i] + [_P] * 6
     lib.ref_zgemv.argtypes = [i, i, i, i] + [_P] * 6 + [d, d]
+    if hasattr(lib, "ref_set_threads"):
+        lib.ref_set_threads.argtypes = [i]
+        lib.ref_set_threads.restype = i
     if hasattr(lib, "ref_zlahqr_eig"):
         lib.ref_zlahqr_eig.argtypes = [i, i, i] + [_P] * 4
         lib.ref_zlahqr_roots.argtypes = [i, i, i] + [_P] * 8
@@ -147,7 +151,10 @@ def ref_eig(A: np.ndarray, thread_cnt: int = 1, clean: bool = True, lib=None):
     rA, iA = to_soa(A, thread_cnt)
     rE = np.zeros(N * n); iE = np.zeros(N * n)
     rV = np.zeros(N * n * n); iV = np.zeros(N * n * n)
-    fn = lib.ref_eigensolver_clean if clean else lib.ref_eigensolver
+    # the as-shipped sequence with zeroed tau/work: the shipped call itself
+    # reads its uninitialised workspace (0 * y in zgemv), see ref_harness.cpp
+    shipped = lib.ref_eigensolver_shipped if hasattr(lib, "ref_eigensolver_shipped") else lib.ref_eigensolver
+    fn = lib.ref_eigensolver_clean if clean else shipped
     fn(n, N // thread_cnt, thread_cnt, dptr(rA), dptr(iA), dptr(rE), dptr(iE), dptr(rV), dptr(iV))
     return (vec_from_soa(rE, iE, n, thread_cnt), from_soa(rV, iV, n, thread_cnt),
             from_soa(rA, iA, n, thread_cnt))
@@ -246,3 +253,24 @@ def eigvec_distance(V1: np.ndarray, V2: np.ndarray, perm: np.ndarray) -> np.ndar
     ip = np.sum(np.conj(W2) * W1, axis=1, keepdims=True)
     ph = ip / np.where(np.abs(ip) == 0, 1, np.abs(ip))
     return np.max(np.linalg.norm(W1 - W2 * ph, axis=1), axis=1)
+
+
+class single_threaded:
+    """Context: the oracle's and the reference build's OpenMP loops on one
+    thread, so that a bit-for-bit comparison of the two compares their
+    arithmetic and not how the batch was scheduled over threads."""
+
+    def __init__(self, orc, ref):
+        self.orc, self.ref, self.prev = orc, ref, None
+
+    def __enter__(self):
+        self.orc.oracle_set_threads(1)
+        if self.ref is not None and hasattr(self.ref, "ref_set_threads"):
+            self.prev = self.ref.ref_set_threads(1)
+        return self
+
+    def __exit__(self, *exc):
+        self.orc.oracle_set_threads(0)
+        if self.prev is not None:
+            self.ref.ref_set_threads(self.prev)
+        return False
