@@ -2,26 +2,31 @@
 // kernels, so that the QR iteration runs on a banded tile and more matrices
 // stay resident per CU.
 //
-//   k_gehrd_c<NB>   the Hessenberg reduction alone (zgehrd, Toolbox.hpp:
-//                   1315-1469) on the dense tile, written in zgehrd's compact
-//                   form -- H on and above the subdiagonal, the reflector
-//                   vectors v(i+2:n) below it, tau(i) -- to the V and E
-//                   planes, which serve as its workspace.
-//   k_qr_band<NB>   Q from those reflectors (zunghr, Toolbox.hpp:1470-1659),
-//                   the QR iteration with Schur vectors (zlahqr_schur,
-//                   Toolbox.hpp:690-1031) and the eigenvectors (ztrevc,
-//                   Toolbox.hpp:1143-1311) on the band row <= col + 2 of the
-//                   tile: every entry a bulge chase, the deflation test and
-//                   the triangular solve touch lies in it.
+//   k_gehrd_c<NB, LW>  the Hessenberg reduction alone (zgehrd, Toolbox.hpp:
+//                      1315-1469) on the dense tile, by one wavefront (NB = 16)
+//                      or two sharing each update (NB = 24..32), written in
+//                      zgehrd's compact form -- H on and above the
+//                      subdiagonal, the reflector vectors v(i+2:n) below it,
+//                      tau(i) -- to the V and E planes, which serve as its
+//                      workspace.
+//   k_qr_band<NB, LW>  Q from those reflectors (zunghr, Toolbox.hpp:
+//                      1470-1659), the QR iteration with Schur vectors
+//                      (zlahqr_schur, Toolbox.hpp:690-1031) and the
+//                      eigenvectors (ztrevc, Toolbox.hpp:1143-1311) on the
+//                      band row <= col + 2 of the tile: every entry a bulge
+//                      chase, the deflation test and the triangular solve
+//                      touch lies in it.
 //
-// At NB = 16 the band holds 166 of the dense tile's 272 entries: 11.8 KB of
-// LDS per wavefront of four matrices instead of 18.6 KB, which with at most
-// 168 registers lets three wavefronts share a SIMD instead of two; at
-// NB = 32, 19.5 KB per wavefront of two instead of 33.8 KB: two wavefronts
-// per SIMD instead of one (at 256 registers, part of the Schur-vector rows
-// spilled).  The
-// reduction round trip through HBM (the compact form written and read back)
-// costs 2 x 16 n^2 bytes per matrix.
+// NB is the slot's capacity, LW the lanes per matrix (0: NB): orders 9..16
+// take <16, 0>, 17..24 <24, 32>, 25..28 <28, 32>, 29..32 <32, 0>.  At NB = 16
+// the band holds 166 of the dense tile's 272 entries: 11.8 KB of LDS per
+// wavefront of four matrices instead of 18.6 KB, which with at most 168
+// registers lets three wavefronts share a SIMD instead of two; with two
+// matrices per wavefront, 11.3 to 19.5 KB instead of 19.2 to 33.8 KB: two
+// wavefronts per SIMD instead of one (the 32-entry Schur-vector rows of
+// NB = 32 partly spilled at 256 registers, the 24-entry ones not).  The
+// reduction's round trip through HBM (the compact form written and read
+// back) costs 2 x 16 n^2 bytes per matrix.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
