@@ -17,8 +17,9 @@
 //                      chase, the deflation test and the triangular solve
 //                      touch lies in it.
 //
-// NB is the slot's capacity, LW the lanes per matrix (0: NB): orders 9..16
-// take <16, 0>, 17..24 <24, 32>, 25..28 <28, 32>, 29..32 <32, 0>.  At NB = 16
+// NB is the slot's capacity, LW the lanes per matrix (0: NB): orders 9..12
+// take <12, 16>, 13..16 <16, 0>, 17..24 <24, 32>, 25..28 <28, 32>, 29..32
+// <32, 0>.  At NB = 16
 // the band holds 166 of the dense tile's 272 entries: 11.8 KB of LDS per
 // wavefront of four matrices instead of 18.6 KB, which with at most 168
 // registers lets three wavefronts share a SIMD instead of two; with two
@@ -101,6 +102,7 @@ int launch_banded(const FusedArgs &a, hipStream_t s) {
   VEIG_HIP(hipGetLastError());
   return VEIG_OK;
 }
+template int launch_banded<12, 16>(const FusedArgs &, hipStream_t);
 template int launch_banded<16, 0>(const FusedArgs &, hipStream_t);
 template int launch_banded<24, 32>(const FusedArgs &, hipStream_t);
 template int launch_banded<28, 32>(const FusedArgs &, hipStream_t);

@@ -27,18 +27,20 @@ struct BandTiles {
 
 // wavefronts per SIMD the QR kernel's register allocation is held to: what
 // the LDS allows (11.8 KB per wavefront at NB = 16, 11.3 KB for two order-24
-// slots, 19.5 KB at NB = 32) where the registers can follow (three at
-// NB = 16; two otherwise, the Schur-vector rows being 24 or 32 entries)
+// slots, 19.5 KB at NB = 32) where the registers can follow (three up to
+// NB = 16; two otherwise, the Schur-vector rows being 24 to 32 entries)
 template <int NB>
-constexpr int qr_band_waves() { return NB == 16 ? 3 : 2; }
+constexpr int qr_band_waves() { return NB <= 16 ? 3 : 2; }
 
 // Wavefronts of a workgroup that share one reduction, and the LDS it needs
 // beside the dense tiles: one partial product per wavefront and lane.  At
 // NB = 16 one wavefront (gehrd_compact) measured faster than two sharing
 // the work (7.6 vs 6.2 ms for 2^20 matrices): the tiles already allow two
 // wavefronts per SIMD there, against one at NB = 32 and half at NB = 64.
+// Slots narrower than their lane group (NB = 12 on 16 lanes) take the
+// shared form too, whose loads and stores skip the lanes past the slot.
 template <int NB>
-constexpr int reduce_waves() { return NB == 64 ? 4 : (NB > 16 ? 2 : 1); }
+constexpr int reduce_waves() { return NB == 64 ? 4 : (NB == 16 ? 1 : 2); }
 
 template <int NB, int LW = 0>
 struct ReduceTiles {

@@ -440,7 +440,7 @@ int launch_zgeev(const FusedArgs &a, hipStream_t s) {
         }
       }
       if constexpr (!f32 && OUT == OUT_V) {
-        if (p == 1 && !a.mask) return launch_banded<16>(a, s);
+        if (p == 1 && !a.mask) return a.G.n <= 12 ? launch_banded<12, 16>(a, s) : launch_banded<16>(a, s);
       }
       if (p == 2) launch_one<R, 16, 2, OUT>(a, s);
       else launch_one<R, 16, 1, OUT>(a, s);
