@@ -20,8 +20,8 @@
 //                  range, read from device memory, wavefront by wavefront; the
 //                  matrices of a wavefront then have (nearly) the same order.
 //                  Bins (16, 16) and (32, 32) run two such kernels instead,
-//                  k_reduce_var and k_qr_var (the banded path); the (32, 32)
-//                  bin's orders <= 24 in slots of capacity 24.
+//                  k_reduce_var and k_qr_var (the banded path); the bins'
+//                  orders <= 12 and <= 24 in slots of capacity 12 and 24.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -35,21 +35,21 @@ using namespace veig;
 namespace {
 
 constexpr int kMaxN = 32;
-// orders of the width-32 bin that run in slots of capacity 24
-constexpr int kSmall = 24;
 // kernel variants (bins): lane-group width NB and slots P, i.e. NB/P lanes
 // per matrix -- the threads-per-matrix choice of the autotuner
 constexpr int kBins = 5;
 __host__ __device__ constexpr int var_nb(int v) { return v < 2 ? 8 : (v < 4 ? 16 : 32); }
 __host__ __device__ constexpr int var_p(int v) { return (v == 1 || v == 3) ? 2 : 1; }
 __host__ __device__ constexpr int default_variant(int n) { return n <= 8 ? 0 : (n <= 16 ? 2 : 4); }
+// orders of a width-16 / width-32 bin that run in slots of capacity 12 / 24
+__host__ __device__ constexpr int small_cut(int v) { return var_nb(v) == 16 ? 12 : (var_nb(v) == 32 ? 24 : 0); }
 
 struct BinState {
   int32_t hist[kMaxN + 1];     // count per order
   int32_t cursor[kMaxN + 1];   // scatter cursor per order
   int64_t start[kBins];        // bin range in the permutation
   int64_t count[kBins];
-  int64_t small[kBins];        // leading part of a width-32 bin with orders <= 24
+  int64_t small[kBins];        // leading part of a bin with orders <= small_cut
   int32_t var_of_n[kMaxN + 1]; // kernel variant per order
 };
 
@@ -108,7 +108,7 @@ __global__ void k_bin_scan(BinState *b) {
       if (b->var_of_n[n] != bi) continue;
       b->cursor[n] = (int32_t)pos;
       pos += b->hist[n];
-      if (n <= kSmall) b->small[bi] = pos - b->start[bi];
+      if (n <= small_cut(bi)) b->small[bi] = pos - b->start[bi];
     }
     b->count[bi] = pos - b->start[bi];
   }
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(64) void k_zgeev_var(VarArgs a, int bin) {
 // banded tile -- the LDS of a wavefront drops from 18.6 to 11.8 KB (NB = 16)
 // and from 33.8 to 19.5 KB (NB = 32), one more resident wavefront per SIMD.
 // Both walk the bin's range like k_zgeev_var.
-// the range of a bin, or of its part 0 (orders <= kSmall) or part 1 (the rest)
+// the range of a bin, or of its part 0 (orders <= small_cut) or part 1 (the rest)
 __device__ __forceinline__ void bin_part(const VarArgs &a, int bin, int part, int64_t &start, int64_t &count) {
   start = a.bins->start[bin];
   count = a.bins->count[bin];
@@ -302,14 +302,17 @@ int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, con
       case 0: hipLaunchKernelGGL((k_zgeev_var<8, 1>), dim3(g), dim3(64), 0, s, a, 0); break;
       case 1: hipLaunchKernelGGL((k_zgeev_var<8, 2>), dim3(g), dim3(64), 0, s, a, 1); break;
       case 2:
-        hipLaunchKernelGGL((k_reduce_var<16, 0>), dim3(g), dim3(64 * reduce_waves<16>()), 0, s, a, 2, -1);
-        hipLaunchKernelGGL((k_qr_var<16, 0>), dim3(g), dim3(64), 0, s, a, 2, -1);
+        // orders <= 12 of the bin in slots of 12, the rest in slots of 16
+        hipLaunchKernelGGL((k_reduce_var<12, 16>), dim3(g), dim3(64 * reduce_waves<12>()), 0, s, a, 2, 0);
+        hipLaunchKernelGGL((k_qr_var<12, 16>), dim3(g), dim3(64), 0, s, a, 2, 0);
+        hipLaunchKernelGGL((k_reduce_var<16, 0>), dim3(g), dim3(64 * reduce_waves<16>()), 0, s, a, 2, 1);
+        hipLaunchKernelGGL((k_qr_var<16, 0>), dim3(g), dim3(64), 0, s, a, 2, 1);
         break;
       case 3: hipLaunchKernelGGL((k_zgeev_var<16, 2>), dim3(g), dim3(64), 0, s, a, 3); break;
       default:
         // orders <= 24 of the bin in slots of 24, the rest in slots of 32
-        hipLaunchKernelGGL((k_reduce_var<kSmall, 32>), dim3(g), dim3(64 * reduce_waves<kSmall>()), 0, s, a, 4, 0);
-        hipLaunchKernelGGL((k_qr_var<kSmall, 32>), dim3(g), dim3(64), 0, s, a, 4, 0);
+        hipLaunchKernelGGL((k_reduce_var<24, 32>), dim3(g), dim3(64 * reduce_waves<24>()), 0, s, a, 4, 0);
+        hipLaunchKernelGGL((k_qr_var<24, 32>), dim3(g), dim3(64), 0, s, a, 4, 0);
         hipLaunchKernelGGL((k_reduce_var<32, 0>), dim3(g), dim3(64 * reduce_waves<32>()), 0, s, a, 4, 1);
         hipLaunchKernelGGL((k_qr_var<32, 0>), dim3(g), dim3(64), 0, s, a, 4, 1);
         break;
