@@ -79,9 +79,23 @@ def timed(fn, steps, warmup, stream):
     return e0.elapsed_time(e1) / steps, time.perf_counter() - t0
 
 
-def roofline(flops_per_launch, bytes_per_launch, kernel_ms, kernel, traffic_key):
+def kernel_sources_sha() -> str:
+    """Hash of the kernel sources (csrc/): traffic figures measured with
+    rocprofv3 are reported only for the build they were measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted((REPO / "vectorized-eigensolver_amd" / "csrc").iterdir()):
+        if f.suffix in (".hip", ".hpp", ".cpp"):
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
+def roofline(flops_per_launch, bytes_per_launch, kernel_ms, kernel, traffic_key, bound="fp64-valu"):
+    """The FP64 kernels run on the vector ALUs (no matrix-core instructions);
+    MI355X's FP64 vector peak equals its FP64 matrix peak, 78.6 TFLOP/s."""
     tf = flops_per_launch / (kernel_ms * 1e-3) / 1e12
-    r = {"bound": "mfma", "achieved": round(tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    r = {"bound": bound, "achieved": round(tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
          "frac": round(tf / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": kernel,
          "kernel_ms": round(kernel_ms, 4),
          "alg_bytes_per_launch": bytes_per_launch,
@@ -89,7 +103,11 @@ def roofline(flops_per_launch, bytes_per_launch, kernel_ms, kernel, traffic_key)
     p = REPO / "profiles" / "traffic.json"
     if p.exists():
         try:
-            r["traffic"] = json.loads(p.read_text()).get(traffic_key, {}).get("bytes_per_launch")
+            rec = json.loads(p.read_text()).get(traffic_key, {})
+            if rec.get("sources_sha") == kernel_sources_sha():
+                r["traffic"] = rec.get("bytes_per_launch")
+            else:
+                r["traffic_note"] = "no rocprofv3 traffic measured on this build of the kernels"
         except Exception:
             pass
     return r
@@ -178,7 +196,7 @@ def bench_uniform(a, cfg, veig, W, torch, dev, stream):
     mixed = None
     if not a.no_mixed:
         progress("timing the mixed-precision path")
-        mixed = bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, v64)
+        mixed = bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, v64, (rE, iE))
 
     flops = N * nominal_flops(n)
     nbytes = N * alg_bytes(n)
@@ -240,7 +258,29 @@ def gather_vec(rX, iX, n, T, S):
     return X.permute(0, 2, 1).reshape(bcs * T, n).contiguous()
 
 
-def bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, fp64_value):
+def eig_err_vs_fp64(rE, iE, rE64, iE64, rA, iA, n, T, N, torch):
+    """Eigenvalue error of the mixed path against the FP64 path on the whole
+    batch: each mixed eigenvalue against the nearest FP64 eigenvalue of its
+    matrix, relative to that eigenvalue and to ||A||_F."""
+    chunk = max(T, (1 << 16) // T * T)
+    worst_rel = worst_nrm = 0.0
+    for s0 in range(0, N, chunk):
+        m = min(chunk, N - s0)
+        E = gather_vec(rE[s0 * n:], iE[s0 * n:], n, T, m)
+        E64 = gather_vec(rE64[s0 * n:], iE64[s0 * n:], n, T, m)
+        D = torch.abs(E[:, :, None] - E64[:, None, :])
+        d, j = D.min(dim=2)
+        lam = torch.gather(E64, 1, j)
+        A = torch.complex(rA[s0 * n * n: (s0 + m) * n * n], iA[s0 * n * n: (s0 + m) * n * n])
+        nrm = torch.linalg.vector_norm(A.view(m // T, n * n, T), dim=1).reshape(-1)  # ||A||_F per matrix
+        worst_rel = max(worst_rel, float((d / torch.abs(lam).clamp_min(1e-300)).max()))
+        worst_nrm = max(worst_nrm, float((d.max(dim=1).values / nrm).max()))
+    return {"max_rel_eig_err_vs_fp64": worst_rel, "max_eig_err_vs_fp64_over_normA": worst_nrm,
+            "eig_err_note": "every matrix of the batch; each mixed eigenvalue against the nearest "
+                            "eigenvalue of the FP64 path's result"}
+
+
+def bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, fp64_value, fp64_E=None):
     N = bc * T
     dev = rA.device
     f64 = dict(dtype=torch.float64, device=dev)
@@ -262,9 +302,10 @@ def bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, fp64_value):
                                        precision="fp32", stream=stream), a.steps, 1, stream)
     step()
     torch.cuda.synchronize()
+    eerr = {} if fp64_E is None else eig_err_vs_fp64(rE, iE, fp64_E[0], fp64_E[1], rA, iA, n, T, N, torch)
     return {"value": round(v, 1), "unit": "matrices/s", "ms_per_step": round(ms, 4),
             "ratio_vs_fp64": round(v / fp64_value, 3),
-            "max_backward_err": float(berr.max()),
+            "max_backward_err": float(berr.max()), **eerr,
             "fp64_fallback_matrices": int(fb.sum()),
             "unconverged_matrices": int((info != 0).sum()),
             "fp32_kernel_ms": round(ms32, 4),
@@ -272,11 +313,38 @@ def bench_mixed(veig, torch, rA, iA, n, bc, T, a, stream, fp64_value):
 
 
 def _cpu_lib():
+    """The reference's own code (oracle/_ref), built for the widest x86-64
+    level this CPU runs (x86-64-v4 = AVX-512 where present, else v3), with
+    its OpenMP threads bound close to the cores; the oracle restatement where
+    the reference build is absent.  Returns (lib, kind, threads)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import helpers
-    lib = helpers.load_ref(exact=False)
-    if lib is not None:
-        return lib, "reference"
-    return helpers.load_oracle(), "port"
+    lib = helpers.load_ref(exact=False, isa="v4" if _cpu_has_v4() else "v3")
+    kind = "reference"
+    if lib is None:
+        lib, kind = helpers.load_oracle(), "port"
+    threads = 0
+    if kind == "reference" and hasattr(lib, "ref_get_threads"):
+        threads = int(lib.ref_get_threads())
+    elif kind == "port" and hasattr(lib, "oracle_get_threads"):
+        threads = int(lib.oracle_get_threads())
+    return lib, kind, threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def _cpu_has_v4() -> bool:
+    try:
+        flags = next(line for line in open("/proc/cpuinfo") if line.startswith("flags")).split()
+    except (OSError, StopIteration):
+        return False
+    return all(f in flags for f in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"))
+
+
+def _median_runs(run, S, reps=5):
+    """Median wall time of `reps` timed runs of run(S), after one warm-up."""
+    import statistics
+    run(S)
+    return statistics.median(run(S) for _ in range(reps))
 
 
 def _run_cpu(lib, kind, n, tc, rA_np, iA_np):
@@ -305,11 +373,12 @@ def _cpu_model():
 
 
 def cpu_baseline_uniform(rA, iA, n, T, N, target_s):
-    """The reference's own OpenMP code (oracle/_ref/libref.so: its four stages
-    with the reflector entries cleared before QR) on a bounded sample of the
-    same inputs; the oracle restatement where the reference build is absent."""
-    lib, kind = _cpu_lib()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    """The reference's own OpenMP code (oracle/_ref/libref*.so: its four
+    stages with the reflector entries cleared before QR) on a bounded sample
+    of the same inputs, sized so that one run takes about target_s / 5; the
+    median of five runs.  The oracle restatement where the reference build is
+    absent."""
+    lib, kind, cores = _cpu_lib()
 
     def run(S):
         bcs = S // T
@@ -320,12 +389,13 @@ def cpu_baseline_uniform(rA, iA, n, T, N, target_s):
     S0 = max(T, (min(N, 4096) // T) * T)
     run(S0)  # warm the thread pool
     t0 = run(S0)
-    S = int(min(N, max(S0, target_s / max(t0, 1e-6) * S0)))
+    S = int(min(N, max(S0, target_s / 5 / max(t0, 1e-6) * S0)))
     S = max(T, (S // T) * T)
-    dt = run(S)
+    dt = _median_runs(run, S)
     return {"value": round(S / dt, 1), "unit": "matrices/s", "cores": cores, "kind": kind,
-            "sample": f"first {S} matrices of the same batch (tile {T}), four stages timed once "
-                      f"after a warm-up call, {dt:.1f} s", "cpu": _cpu_model()}
+            "sample": f"first {S} matrices of the same batch (tile {T}), four stages, median of 5 runs "
+                      f"after a warm-up run, {dt:.2f} s per run; OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}",
+            "cpu": _cpu_model()}
 
 
 def bench_roots(a, cfg, veig, W, torch, dev, stream):
@@ -421,7 +491,7 @@ def bench_sweep(a, cfg, veig, W, torch, dev, stream):
     if not a.no_cpu:
         # the reference's own OpenMP code on the same 13,312-matrix batches at
         # four of the orders (whole batch, timed once after a warm-up call)
-        lib, kind = _cpu_lib()
+        lib, kind, cores = _cpu_lib()
         per = {}
         for n in (16, 32, 48, 64):
             if not cfg["nmin"] <= n <= cfg["nmax"]:
@@ -432,8 +502,7 @@ def bench_sweep(a, cfg, veig, W, torch, dev, stream):
             dt = _run_cpu(lib, kind, n, T, a_np.copy(), b_np.copy())
             per[n] = round(N / dt, 1)
             progress(f"sweep CPU reference n={n}: {per[n]:.0f} matrices/s")
-        cpu = {"per_order": per, "unit": "matrices/s", "kind": kind,
-               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+        cpu = {"per_order": per, "unit": "matrices/s", "kind": kind, "cores": cores,
                "sample": f"the full {N}-matrix batch at orders {sorted(per)}", "cpu": _cpu_model()}
     return {"metric": f"eigendecomp matrices/s, orders {cfg['nmin']}..{cfg['nmax']}, {N} matrices per order",
             "value": round(value, 1), "unit": "matrices/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -450,8 +519,7 @@ def cpu_baseline_roots(rC, iC, n, T, N, target_s):
     where the reference build is absent."""
     import numpy as np
     import helpers
-    lib, kind = _cpu_lib()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    lib, kind, cores = _cpu_lib()
 
     def run(S):
         bcs = S // T
@@ -469,12 +537,12 @@ def cpu_baseline_roots(rC, iC, n, T, N, target_s):
     S0 = max(T, (min(N, 4096) // T) * T)
     run(S0)
     t0 = run(S0)
-    S = int(min(N, max(S0, target_s / max(t0, 1e-6) * S0)))
+    S = int(min(N, max(S0, target_s / 5 / max(t0, 1e-6) * S0)))
     S = max(T, (S // T) * T)
-    dt = run(S)
+    dt = _median_runs(run, S)
     return {"value": round(S / dt, 1), "unit": "polynomials/s", "cores": cores, "kind": kind,
             "sample": f"first {S} polynomials of the same batch (tile {T}), the reference's zlahqr roots "
-                      f"kernel (Schur form and vectors included) timed once after a warm-up call, {dt:.1f} s",
+                      f"kernel (Schur form and vectors included), median of 5 runs, {dt:.2f} s per run",
             "cpu": _cpu_model()}
 
 
@@ -544,8 +612,7 @@ def cpu_baseline_varsize(vb, dims, target_s):
     """The reference's code on a sample of the batch: it only takes
     equal-size batches, so the sample is grouped by order (thread_cnt 1)."""
     import numpy as np
-    lib, kind = _cpu_lib()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    lib, kind, cores = _cpu_lib()
     d = dims.cpu().numpy()
     a_off = vb.a_off.cpu().numpy()
     rA = vb.rA.cpu().numpy()
@@ -562,10 +629,11 @@ def cpu_baseline_varsize(vb, dims, target_s):
     S0 = min(len(d), 4096)
     run(S0)
     t0 = run(S0)
-    S = int(min(len(d), max(S0, target_s / max(t0, 1e-6) * S0)))
-    dt = run(S)
+    S = int(min(len(d), max(S0, target_s / 5 / max(t0, 1e-6) * S0)))
+    dt = _median_runs(run, S)
     return {"value": round(S / dt, 1), "unit": "matrices/s", "cores": cores, "kind": kind,
-            "sample": f"first {S} matrices of the batch, grouped by order (thread_cnt 1), {dt:.1f} s",
+            "sample": f"first {S} matrices of the batch, grouped by order (thread_cnt 1), median of 5 runs, "
+                      f"{dt:.2f} s per run",
             "cpu": _cpu_model()}
 
 

@@ -43,10 +43,16 @@ typedef enum {
 /* flags for veig_zgeev */
 #define VEIG_WRITE_SCHUR 1u /* also store the Schur form T over A (zeros below the diagonal) */
 
-/* Largest order of the fused entries (veig_zgeev, veig_zgeev_fp32,
- * veig_zgeev_mixed, veig_host_eigensolver) and of the other entries. */
-#define VEIG_MAX_DIM 64
-#define VEIG_MAX_DIM_STAGES 32
+/* Largest orders.  VEIG_MAX_DIM: veig_zgeev (FP64), veig_host_eigensolver and
+ * the stage entries (orders above 64, and above 32 at the stage entries, run
+ * on one workgroup per matrix with the matrix in LDS: 96 x 97 complex doubles
+ * fill the LDS of a CU).  VEIG_MAX_DIM_FAST: veig_zgeev_fp32,
+ * veig_zgeev_mixed, the eigenvalues-only and roots entries.
+ * VEIG_MAX_DIM_VARSIZE: veig_zgeev_varsize (larger orders get info = -1). */
+#define VEIG_MAX_DIM 96
+#define VEIG_MAX_DIM_STAGES VEIG_MAX_DIM
+#define VEIG_MAX_DIM_FAST 64
+#define VEIG_MAX_DIM_VARSIZE 32
 
 const char *veig_status_string(int status);
 const char *veig_last_error(void);
@@ -65,22 +71,24 @@ const char *veig_version(void);
  * Toolbox.hpp:808, and reports nothing; itmax <= 0 selects 30).
  * A, E and V are separate arrays: at orders 9..64 the call passes the
  * Hessenberg reduction's compact form between its kernels through V and E,
- * which then receive the results.
+ * which then receive the results; above 64, V holds Q and then the Schur
+ * vectors on the way to the eigenvectors.
  * ------------------------------------------------------------------------- */
 int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
                double *rE, double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
                uint32_t flags, void *stream);
 
 /* Same with the reduction, QR and eigenvector stages in complex64 (inputs
- * and outputs stay complex128).  No refinement: the building block of the
- * mixed-precision path, also usable on its own. */
+ * and outputs stay complex128), orders up to VEIG_MAX_DIM_FAST.  No
+ * refinement: the building block of the mixed-precision path, also usable on
+ * its own. */
 int veig_zgeev_fp32(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA,
                     const double *iA, double *rE, double *iE, double *rV, double *iV,
                     int32_t *info, int32_t itmax, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Mixed precision.  No counterpart in the reference (new in this design, for
- * the same eigendecomposition as veig_zgeev, orders up to VEIG_MAX_DIM): the
+ * the same eigendecomposition as veig_zgeev, orders up to VEIG_MAX_DIM_FAST): the
  * reduction, QR sweeps and eigenvector solves run in complex64; every eigenpair is then refined in
  * FP64 by Newton's method on R = A V - V diag(E) until the backward error
  * ||R||_F / (||A||_F ||V||_F) <= tol (at most 3 corrections); matrices that do

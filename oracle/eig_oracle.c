@@ -636,6 +636,7 @@ static void ztrevc_one(mat_t Tm, mat_t R, double *rwork, double *iwork, int flag
 
 static int g_threads = 0;
 void oracle_set_threads(int nthreads) { g_threads = nthreads; }
+int oracle_get_threads(void) { return g_threads > 0 ? g_threads : omp_get_max_threads(); }
 
 static mat_t slot(double *re, double *im, int n, int T, int b, int t) {
   mat_t M;

@@ -86,6 +86,7 @@ void oracle_backsubstitution(int dim, int block_cnt, int thread_cnt, const doubl
                              const double *iT, double *rb, double *ib);
 /* Threads the OpenMP loops use (0 = library default). */
 void oracle_set_threads(int nthreads);
+int oracle_get_threads(void);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,9 @@ int ref_set_threads(int nthreads) {
   return prev;
 }
 
+// Threads the reference's OpenMP loops use for calls from this thread.
+int ref_get_threads(void) { return omp_get_max_threads(); }
+
 // The reference's fused host call, exactly as shipped.
 void ref_eigensolver(int dim, int block_cnt, int thread_cnt, double *rA, double *iA, double *rE,
                      double *iE, double *rV, double *iV) {

@@ -19,6 +19,7 @@ ORACLE_DIR = REPO / "oracle"
 ORACLE_SO = ORACLE_DIR / "liboracle.so"
 REF_SO = ORACLE_DIR / "_ref" / "libref.so"
 REF_EXACT_SO = ORACLE_DIR / "_ref" / "libref_exact.so"
+REF_V4_SO = ORACLE_DIR / "_ref" / "libref_v4.so"
 
 # oracle behaviour switches (oracle/eig_oracle.h)
 KEEP_TRASH = 1
@@ -96,14 +97,19 @@ def load_oracle():
     lib.oracle_zgemv.argtypes = [i, i, i, i] + [_P] * 6 + [d, d, i]
     lib.oracle_backsubstitution.argtypes = [i, i, i] + [_P] * 4
     lib.oracle_set_threads.argtypes = [i]
+    if hasattr(lib, "oracle_get_threads"):
+        lib.oracle_get_threads.restype = i
     lib.oracle_zlahqr_eig.argtypes = [i, i, i] + [_P] * 4 + [i, ctypes.POINTER(_Stats)]
     lib.oracle_companion.argtypes = [i, i, i] + [_P] * 4
     lib.oracle_zlahqr_roots.argtypes = [i, i, i] + [_P] * 8 + [i, ctypes.POINTER(_Stats)]
     return lib
 
 
-def load_ref(exact: bool = True):
-    path = REF_EXACT_SO if exact else REF_SO
+def load_ref(exact: bool = True, isa: str = "v3"):
+    """The reference checker build: exact (FMA contraction off, x86-64-v3)
+    or as the reference builds itself (contraction on), for x86-64-v3 or,
+    with isa="v4", x86-64-v4 (AVX-512)."""
+    path = REF_EXACT_SO if exact else (REF_V4_SO if isa == "v4" and REF_V4_SO.exists() else REF_SO)
     if not path.exists():
         return None
     lib = ctypes.CDLL(str(path))
@@ -119,6 +125,8 @@ def load_ref(exact: bool = True):
     if hasattr(lib, "ref_set_threads"):
         lib.ref_set_threads.argtypes = [i]
         lib.ref_set_threads.restype = i
+    if hasattr(lib, "ref_get_threads"):
+        lib.ref_get_threads.restype = i
     if hasattr(lib, "ref_zlahqr_eig"):
         lib.ref_zlahqr_eig.argtypes = [i, i, i] + [_P] * 4
         lib.ref_zlahqr_roots.argtypes = [i, i, i] + [_P] * 8

@@ -35,9 +35,11 @@ def test_argument_validation_without_device():
     z = [None] * 6
     assert L.veig_zgeev(-1, 1, 1, *z, None, 30, 0, None) == 1
     assert L.veig_zgeev(8, 1, 0, *z, None, 30, 0, None) == 1
-    assert L.veig_zgeev(65, 1, 1, *z, None, 30, 0, None) == 2
-    assert b"exceeds 64" in L.veig_last_error()
-    assert L.veig_zgehrd(33, 1, 1, None, None, None, None, None) == 2   # stage entries: n <= 32
+    assert L.veig_zgeev(97, 1, 1, *z, None, 30, 0, None) == 2
+    assert b"exceeds 96" in L.veig_last_error()
+    assert L.veig_zgehrd(97, 1, 1, None, None, None, None, None) == 2   # stage entries: n <= 96
+    assert L.veig_zgehrd(33, 1, 1, None, None, None, None, None) == 1   # ... then the NULL pointers
+    assert L.veig_zgeev_fp32(65, 1, 1, *z, None, 30, None) == 2         # complex64: n <= 64
     assert L.veig_zgeev(8, 2, 2, *z, None, 30, 0, None) == 1
     assert b"NULL pointer" in L.veig_last_error()
     # empty batches are a successful no-op (nothing to do, no device needed)

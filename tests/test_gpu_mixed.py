@@ -123,5 +123,12 @@ def test_mixed_workspace_checks(gpu):
     dev = torch.device("cuda:0")
     trA, tiA = torch.from_numpy(rA).to(dev), torch.from_numpy(iA).to(dev)
     small = torch.empty(16, dtype=torch.uint8, device=dev)
-    with pytest.raises(veig.VeigError, match="workspace too small"):
+    # the binding refuses it before the call ...
+    with pytest.raises(veig.VeigError, match="workspace: 16 elements"):
         veig.zgeev_mixed(trA, tiA, 8, 8, 1, workspace=small)
+    # ... and so does the library itself
+    out = [torch.empty(64 * 8, dtype=torch.float64, device=dev) for _ in range(2)] + \
+          [torch.empty(64 * 64, dtype=torch.float64, device=dev) for _ in range(2)]
+    st = veig.lib().veig_zgeev_mixed(8, 8, 1, veig._ptr(trA), veig._ptr(tiA), *[veig._ptr(t) for t in out],
+                                     None, None, None, 1e-12, 30, veig._ptr(small), 16, veig._stream(None))
+    assert st == 1 and b"workspace too small" in veig.lib().veig_last_error()

@@ -14,7 +14,10 @@ pytestmark = pytest.mark.gpu
 
 
 def build(dims, seed=0):
-    """VarBatch with a Gaussian/Hermitian/constructed mix; returns (vb, mats)."""
+    """VarBatch with the mix of the uneven-order workload (262,144 matrices,
+    n in [4, 32]): half Gaussian, a quarter Hermitian, a quarter
+    V diag(lam) V^-1 with cond(V) = 1e2 and lam in clusters 1e-6 apart;
+    returns (vb, mats)."""
     rng = np.random.default_rng(seed)
     vb = veig.VarBatch(dims)
     mats = []
@@ -28,7 +31,7 @@ def build(dims, seed=0):
         kind = m % 4
         s = int(rng.integers(1 << 30))
         A = (gaussian(1, n, s) if kind < 2 else hermitian(1, n, s) if kind == 2
-             else constructed(1, n, s, cond=1e2))[0]
+             else constructed(1, n, s, cond=1e2, cluster=1e-6))[0]
         mats.append(A)
         cm = A.T.reshape(-1)
         rA[a_off[m]:a_off[m] + n * n] = cm.real

@@ -41,7 +41,7 @@ def build_lib(verbose: bool = False) -> Path:
 
     srcs = [CSRC / "veig.hip", CSRC / "veig_mixed.hip", CSRC / "veig_varsize.hip",
             CSRC / "veig_large.hip", CSRC / "veig_roots.hip", CSRC / "veig_band.hip",
-            CSRC / "veig_compat.cpp"]
+            CSRC / "veig_big.hip", CSRC / "veig_compat.cpp"]
     srcs = [s for s in srcs if s.exists()]
     objdir = REPO / "_build"
     objdir.mkdir(exist_ok=True)
@@ -76,6 +76,25 @@ def build_lib(verbose: bool = False) -> Path:
     return LIB
 
 
+def build_probes(verbose: bool = False) -> None:
+    """The test probe of the scalar complex API (include/veig_complex.hpp):
+    a host build (g++) and a gfx950 build (hipcc) of tests/probe/."""
+    probe = REPO / "tests" / "probe"
+    cmds = [
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", str(REPO / "include"),
+         "-o", str(probe / "libcomplex_probe_host.so"), str(probe / "complex_probe_host.cpp")],
+        [hipcc(), *HIPCC_FLAGS, "-I", str(REPO / "include"),
+         "-o", str(probe / "libcomplex_probe.so"), str(probe / "complex_probe.hip")],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"probe build failed: {cmd[-1]}")
+
+
 def build_oracle(verbose: bool = False) -> None:
     targets = ["all"]
     if Path("/root/reference/utils.cpp").exists():
@@ -99,7 +118,8 @@ def main() -> None:
         print("built", build_lib(a.v))
     if not a.lib:
         build_oracle(a.v)
-        print("built oracle")
+        build_probes(a.v)
+        print("built oracle and test probes")
 
 
 if __name__ == "__main__":

@@ -17,17 +17,6 @@ std::string &last_error() {
   return s;
 }
 
-int slots_for(bool fp32, int nb) {
-  // more matrices per wavefront pay while the slot registers fit
-  static const int env = [] {
-    const char *e = getenv("VEIG_SLOTS");
-    return e ? atoi(e) : 0;
-  }();
-  if (env == 1 || env == 2 || env == 4) return env;
-  (void)fp32;
-  (void)nb;
-  return 1;
-}
 }  // namespace veig
 
 using namespace veig;
@@ -70,7 +59,7 @@ __global__ __launch_bounds__(64) void k_stage(StageArgs a) {
     cx<R> tau = mk<R>(0, 0);
     gehrd_q<R, NB>(g, z, nmax, a.mode == ST_HESS, true, &tau);
     store_rows<R, NB>(g, a.G, base, a.rA, a.iA);
-    if (g.r + 1 < g.n) {
+    if (a.rE && g.r + 1 < g.n) {
       a.rE[vbase + a.G.ve(g.r)] = tau.re;
       a.iE[vbase + a.G.ve(g.r)] = tau.im;
     }
@@ -172,7 +161,34 @@ __global__ void k_backsub(Geom G, const double *rT, const double *iT, double *rb
   }
 }
 
-// layout converters: one thread per element of the output
+// AoS <-> tiled SoA.  Within a block of T matrices the AoS plane is a
+// T x n^2 row-major array ([t][e]) and the SoA plane its transpose ([e][t]),
+// so each conversion is a batch of transposes of rows x cols arrays (AoS ->
+// SoA: T x n^2; SoA -> AoS: n^2 x T), every block at offset b T n^2 in both.
+// One workgroup moves a 64 x 64 tile through LDS: reads along the input's
+// rows, writes along the output's rows, both coalesced.
+constexpr int kTr = 64;
+__global__ __launch_bounds__(256) void k_transpose(int64_t rows, int64_t cols, const double *in, double *out) {
+  __shared__ double tile[kTr][kTr + 1];
+  const int64_t ntr = (rows + kTr - 1) / kTr, ntc = (cols + kTr - 1) / kTr;
+  const int64_t id = blockIdx.x, per = ntr * ntc;
+  const int64_t b = id / per, rem = id - b * per;
+  const int64_t r0 = rem / ntc * kTr, c0 = rem % ntc * kTr;
+  const int64_t off = b * rows * cols;
+  const int tx = threadIdx.x % kTr, ty = threadIdx.x / kTr;
+  for (int i = ty; i < kTr; i += 256 / kTr) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r < rows && c < cols) tile[i][tx] = in[off + r * cols + c];
+  }
+  __syncthreads();
+  for (int i = ty; i < kTr; i += 256 / kTr) {
+    const int64_t c = c0 + i, r = r0 + tx;  // output row c, column r
+    if (c < cols && r < rows) out[off + c * rows + r] = tile[tx][i];
+  }
+}
+
+// replication (and the T = 1 conversions, plain copies): one thread per
+// element of the output
 __global__ void k_layout(int dir, Geom G, const double *in, double *out, int64_t total) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= total) return;
@@ -253,7 +269,7 @@ const char *veig_status_string(int status) {
 
 const char *veig_last_error(void) { return last_error().c_str(); }
 
-const char *veig_version(void) { return "veig 0.2 gfx950 lane-group kernels (fused n <= 64, stages n <= 32)"; }
+const char *veig_version(void) { return "veig 0.3 gfx950 lane-group kernels (fused and stages n <= 96)"; }
 
 int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
                double *rE, double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
@@ -269,7 +285,7 @@ int veig_zgeev(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, d
 int veig_zgeev_fp32(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA,
                     const double *iA, double *rE, double *iE, double *rV, double *iV,
                     int32_t *info, int32_t itmax, void *stream) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM_FAST));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV); VEIG_REQUIRE(iV);
   FusedArgs a{const_cast<double *>(rA), const_cast<double *>(iA), rE, iE, rV, iV, info,
@@ -282,7 +298,9 @@ int veig_zgehrd(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, 
   VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rtau); VEIG_REQUIRE(itau);
-  StageArgs a{rA, iA, nullptr, nullptr, rtau, itau, nullptr, make_geom(dim, block_cnt, thread_cnt), 30, ST_GEHRD};
+  const Geom G = make_geom(dim, block_cnt, thread_cnt);
+  if (dim > 32) return launch_big(BIG_GEHRD, G, rA, iA, nullptr, nullptr, rtau, itau, nullptr, 30, 0, (hipStream_t)stream);
+  StageArgs a{rA, iA, nullptr, nullptr, rtau, itau, nullptr, G, 30, ST_GEHRD};
   return launch_stage(a, (hipStream_t)stream);
 }
 
@@ -291,17 +309,12 @@ int veig_hessenberg(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *
   VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rQ); VEIG_REQUIRE(iQ);
-  // tau is produced and consumed on chip; the launch still needs a place
-  // for it, which is the sub-diagonal-free part of nothing: reuse a scratch
-  // allocation (the only allocating device entry).
-  const size_t sv = (size_t)dim * block_cnt * thread_cnt;
-  DevBuf rt, it;
-  VEIG_TRY(dev_alloc(rt, sv));
-  VEIG_TRY(dev_alloc(it, sv));
-  StageArgs a{rA, iA, rQ, iQ, rt.p, it.p, nullptr, make_geom(dim, block_cnt, thread_cnt), 30, ST_HESS};
-  VEIG_TRY(launch_stage(a, (hipStream_t)stream));
-  VEIG_HIP(hipStreamSynchronize((hipStream_t)stream));
-  return VEIG_OK;
+  // tau is produced and consumed on chip (Q is accumulated forward), so no
+  // tau planes are needed: the kernels skip writing it
+  const Geom G = make_geom(dim, block_cnt, thread_cnt);
+  if (dim > 32) return launch_big(BIG_HESS, G, rA, iA, rQ, iQ, nullptr, nullptr, nullptr, 30, 0, (hipStream_t)stream);
+  StageArgs a{rA, iA, rQ, iQ, nullptr, nullptr, nullptr, G, 30, ST_HESS};
+  return launch_stage(a, (hipStream_t)stream);
 }
 
 int veig_zunghr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, double *iA,
@@ -309,8 +322,10 @@ int veig_zunghr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rA, 
   VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rtau); VEIG_REQUIRE(itau);
-  StageArgs a{rA, iA, nullptr, nullptr, const_cast<double *>(rtau), const_cast<double *>(itau), nullptr,
-              make_geom(dim, block_cnt, thread_cnt), 30, ST_UNGHR};
+  const Geom G = make_geom(dim, block_cnt, thread_cnt);
+  double *rt = const_cast<double *>(rtau), *it = const_cast<double *>(itau);
+  if (dim > 32) return launch_big(BIG_UNGHR, G, rA, iA, nullptr, nullptr, rt, it, nullptr, 30, 0, (hipStream_t)stream);
+  StageArgs a{rA, iA, nullptr, nullptr, rt, it, nullptr, G, 30, ST_UNGHR};
   return launch_stage(a, (hipStream_t)stream);
 }
 
@@ -320,8 +335,10 @@ int veig_zlahqr(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rE, 
   VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rH); VEIG_REQUIRE(iH); VEIG_REQUIRE(rZ); VEIG_REQUIRE(iZ);
-  StageArgs a{rH, iH, rZ, iZ, rE, iE, info, make_geom(dim, block_cnt, thread_cnt), itmax > 0 ? itmax : 30,
-              ST_LAHQR};
+  const Geom G = make_geom(dim, block_cnt, thread_cnt);
+  const int its = itmax > 0 ? itmax : 30;
+  if (dim > 32) return launch_big(BIG_LAHQR, G, rH, iH, rZ, iZ, rE, iE, info, its, 0, (hipStream_t)stream);
+  StageArgs a{rH, iH, rZ, iZ, rE, iE, info, G, its, ST_LAHQR};
   return launch_stage(a, (hipStream_t)stream);
 }
 
@@ -330,8 +347,10 @@ int veig_ztrevc(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double
   VEIG_TRY(check_geom(dim, block_cnt, thread_cnt));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rT); VEIG_REQUIRE(iT); VEIG_REQUIRE(rR); VEIG_REQUIRE(iR);
-  StageArgs a{const_cast<double *>(rT), const_cast<double *>(iT), rR, iR, nullptr, nullptr, nullptr,
-              make_geom(dim, block_cnt, thread_cnt), 30, ST_TREVC};
+  const Geom G = make_geom(dim, block_cnt, thread_cnt);
+  double *rt = const_cast<double *>(rT), *it = const_cast<double *>(iT);
+  if (dim > 32) return launch_big(BIG_TREVC, G, rt, it, rR, iR, nullptr, nullptr, nullptr, 30, 0, (hipStream_t)stream);
+  StageArgs a{rt, it, rR, iR, nullptr, nullptr, nullptr, G, 30, ST_TREVC};
   return launch_stage(a, (hipStream_t)stream);
 }
 
@@ -367,6 +386,15 @@ static int layout(int dir, int64_t dim, int64_t bc, int64_t tc, const double *in
   if (empty_batch(dim, bc, tc)) return VEIG_OK;
   VEIG_REQUIRE(in); VEIG_REQUIRE(out);
   const Geom G = make_geom(dim, bc, tc);
+  if (bc > 0 && tc > INT64_MAX / bc / (dim * dim + 1)) return fail(VEIG_ERR_INVALID, "batch size overflows");
+  if (dir <= 1 && tc > 1) {
+    const int64_t n2 = dim * dim, rows = dir == 0 ? tc : n2, cols = dir == 0 ? n2 : tc;
+    const int64_t tiles = bc * ((rows + kTr - 1) / kTr) * ((cols + kTr - 1) / kTr);
+    if (tiles > INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");
+    hipLaunchKernelGGL(k_transpose, dim3((unsigned)tiles), dim3(256), 0, (hipStream_t)stream, rows, cols, in, out);
+    VEIG_HIP(hipGetLastError());
+    return VEIG_OK;
+  }
   const int64_t total = (dir <= 2) ? G.nmat * dim * dim : G.nmat * dim;
   const int64_t grid = (total + 255) / 256;
   if (grid > INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");

@@ -388,11 +388,6 @@ __global__ __launch_bounds__(64) void k_zgeev_list(FusedArgs a) {
   }
 }
 
-// Lane slots per index of the fused kernel, per precision and padded order
-// (P = 1: one lane per index).  VEIG_SLOTS=<p> overrides the table where the
-// kernel exists with that many slots (tuning).
-int slots_for(bool fp32, int nb);
-
 // persistent grid of list launches (the list length is only known on the device)
 constexpr int64_t kListGrid = 4096;
 
@@ -415,48 +410,53 @@ inline void launch_one(const FusedArgs &a, hipStream_t s) {
 template <typename R, int OUT>
 int launch_zgeev64(const FusedArgs &a, hipStream_t s);
 
+// Orders beyond the lane-group kernels (veig_big.hip): one workgroup per
+// matrix, the matrix in LDS, the unitary factor in global memory.  Modes:
+// the fused eigendecomposition (A read only; Q, Z and then V in B; E), and
+// the stage entries in place: zgehrd (A -> compact form, tau in E),
+// hessenberg (also Q in B), zunghr (reflectors in A, tau in E -> Q in A),
+// zlahqr (H in A -> T, Z in B updated, E), ztrevc (T in A, B <- B X).
+enum { BIG_FUSED = 0, BIG_GEHRD = 1, BIG_HESS = 2, BIG_UNGHR = 3, BIG_LAHQR = 4, BIG_TREVC = 5 };
+int launch_big(int mode, const Geom &G, double *rA, double *iA, double *rB, double *iB, double *rE, double *iE,
+               int32_t *info, int itmax, uint32_t flags, hipStream_t s);
+
 // FP64 batches and lists of orders up to NB on lane groups of LW lanes
 // (0: NB): Hessenberg reduction, then the QR iteration and eigenvectors on
 // a banded tile (two kernels)
 template <int NB, int LW = 0>
 int launch_banded(const FusedArgs &a, hipStream_t s);
 
+// One index per lane (P = 1) everywhere: two or four indices per lane put
+// more matrices in a wavefront but measured slower at every order (at n = 16
+// FP64 the slot registers drop the occupancy to one wavefront per SIMD).
 template <typename R, int OUT = OUT_V>
 int launch_zgeev(const FusedArgs &a, hipStream_t s) {
   constexpr bool f32 = std::is_same<R, float>::value;
-  const int nb = nb_for(a.G.n);
-  const int p = slots_for(f32, nb);
   if (a.G.nmat > (int64_t)INT32_MAX) return fail(VEIG_ERR_INVALID, "batch too large for one launch");
+  if (a.G.n > 64) {
+    if constexpr (!f32 && OUT == OUT_V) {
+      if (!a.mask && !a.list)
+        return launch_big(BIG_FUSED, a.G, a.rA, a.iA, a.rV, a.iV, a.rE, a.iE, a.info, a.itmax, a.flags, s);
+    }
+    return fail(VEIG_ERR_UNSUPPORTED, "orders above 64: FP64 whole batches only");
+  }
+  const int nb = nb_for(a.G.n);
   switch (nb) {
     case 8:
-      if (p == 2) launch_one<R, 8, 2, OUT>(a, s);
-      else launch_one<R, 8, 1, OUT>(a, s);
+      launch_one<R, 8, 1, OUT>(a, s);
       break;
     case 16:
-      if constexpr (f32) {
-        if (p == 4) {
-          launch_one<R, 16, 4, OUT>(a, s);
-          break;
-        }
-      }
       if constexpr (!f32 && OUT == OUT_V) {
-        if (p == 1 && !a.mask) return a.G.n <= 12 ? launch_banded<12, 16>(a, s) : launch_banded<16>(a, s);
+        if (!a.mask) return a.G.n <= 12 ? launch_banded<12, 16>(a, s) : launch_banded<16>(a, s);
       }
-      if (p == 2) launch_one<R, 16, 2, OUT>(a, s);
-      else launch_one<R, 16, 1, OUT>(a, s);
+      launch_one<R, 16, 1, OUT>(a, s);
       break;
     case 32:
       if constexpr (!f32 && OUT == OUT_V) {
         // slots of capacity 24 or 28 where they fit: shorter Schur-vector rows, fewer spills
-        if (p == 1 && !a.mask)
+        if (!a.mask)
           return a.G.n <= 24 ? launch_banded<24, 32>(a, s)
                              : (a.G.n <= 28 ? launch_banded<28, 32>(a, s) : launch_banded<32>(a, s));
-      }
-      if constexpr (f32) {
-        if (p == 2) {
-          launch_one<R, 32, 2, OUT>(a, s);
-          break;
-        }
       }
       launch_one<R, 32, 1, OUT>(a, s);
       break;

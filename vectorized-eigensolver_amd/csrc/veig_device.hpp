@@ -481,32 +481,106 @@ __device__ __forceinline__ void larfg2(cx<R> a0, cx<R> a1, cx<R>& t1, cx<R>& v2,
   beta_out = mk<R>(-beta, 0);
 }
 
+// A bulge-chase reflector G = I - t1 [1; v2][1; v2]^H, with t2 = t1 v2,
+// which the chase keeps real.
+template <typename R>
+struct Refl {
+  cx<R> t1, v2;
+  R t2;
+};
+
+// The reflector of zlarfg with n = 2 (LAPACK sign, Toolbox.hpp:183-242) for
+// [a0; a1] with a1 real: the first step's a1 is the scaled subdiagonal entry,
+// and every later one is the bulge -t2 H(K+1, K) of a real subdiagonal entry
+// (the reference's own remark, Toolbox.hpp:172-174).  With
+// beta = copysign(|[a0; a1]|, Re a0):
+//   t1 = (a0 + beta) / beta,  t2 = t1 v2 = a1 / beta,
+//   v2 = a1 / (a0 + beta) = t2 conj(t1) / |t1|^2,  |t1|^2 = 2 Re t1 - t2^2
+// (the last from G being unitary; it lies in [1, 4], so the subtraction does
+// not cancel): one reciprocal square root and one reciprocal.  beta_out =
+// -beta is the new a0.  The zero vector (a1 = 0 = Im a0), where zlarfg
+// returns the identity, is left to the caller.
+template <typename R>
+__device__ __forceinline__ Refl<R> larfg2r(cx<R> a0, R a1, R& beta_out) {
+  const R nrm2 = fma(a0.re, a0.re, fma(a0.im, a0.im, a1 * a1));
+  const R ib = __builtin_copysign(frsq(nrm2), a0.re);  // 1/beta
+  beta_out = -(nrm2 * ib);
+  Refl<R> q;
+  q.t1 = mk(fma(a0.re, ib, R(1)), a0.im * ib);
+  q.t2 = a1 * ib;
+  const R s = q.t2 * frcp(fma(-q.t2, q.t2, q.t1.re + q.t1.re));
+  q.v2 = mk(q.t1.re * s, -q.t1.im * s);
+  return q;
+}
+
+// [h1; h2] <- G^H [h1; h2]: the reflector from the left (rows K, K+1);
+// s = conj(t1) h1 + t2 h2, h1 -= s, h2 -= s v2
+template <typename R>
+__device__ __forceinline__ void refl_left(const Refl<R>& q, cx<R>& h1, cx<R>& h2) {
+  const R sr = fma(q.t1.re, h1.re, fma(q.t1.im, h1.im, q.t2 * h2.re));
+  const R si = fma(q.t1.re, h1.im, fma(-q.t1.im, h1.re, q.t2 * h2.im));
+  h1.re -= sr;
+  h1.im -= si;
+  h2.re = fma(-sr, q.v2.re, fma(si, q.v2.im, h2.re));
+  h2.im = fma(-sr, q.v2.im, fma(-si, q.v2.re, h2.im));
+}
+
+// [h1, h2] <- [h1, h2] G: the reflector from the right (columns K, K+1 of H
+// and of the Schur vectors); s = t1 h1 + t2 h2, h1 -= s, h2 -= s conj(v2)
+template <typename R>
+__device__ __forceinline__ void refl_right(const Refl<R>& q, cx<R>& h1, cx<R>& h2) {
+  const R sr = fma(q.t1.re, h1.re, fma(-q.t1.im, h1.im, q.t2 * h2.re));
+  const R si = fma(q.t1.re, h1.im, fma(q.t1.im, h1.re, q.t2 * h2.im));
+  h1.re -= sr;
+  h1.im -= si;
+  h2.re = fma(-sr, q.v2.re, fma(-si, q.v2.im, h2.re));
+  h2.im = fma(-si, q.v2.re, fma(sr, q.v2.im, h2.im));
+}
+
 // Apply one bulge-chase reflector at row K to this lane's slots: rows K, K+1
 // of H over the lane's columns >= K, columns K, K+1 of H over the lane's rows
 // <= i3, and columns K, K+1 of the Schur vectors.  KC >= 0 is K as a
 // compile-time constant (unrolled sweeps); KC < 0 takes the wavefront-uniform
 // runtime Krt and selects the register columns by a uniform branch tree.
+// The reflector's input is column K-1, rows K and K+1; at the first step of
+// a matrix's sweep (K = L) the caller has staged the shifted first column
+// (a0; a1 real) there -- in vsh[0..1] when L = 0 -- so that every step reads
+// its input from a fixed place, without a select or a branch.
 // TZ = false is the eigenvalues-only sweep (single_shift_qr, Toolbox.hpp:
 // 372-522): H is updated inside the active block L..I only (columns <= I,
 // rows >= L) and there are no Schur vectors.
 template <int KC, bool TZ = true, typename R, int NB, int P, int LAY, int LW>
 __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], bool act,
-                                           int L, int I, cx<R> a0, cx<R> a1, int Krt = 0) {
+                                           int L, int I, int Krt = 0) {
   using L2 = typename Group<R, NB, P, LAY, LW>::L2;
   constexpr int LG = Group<R, NB, P, LAY, LW>::LG;
   const int n = g.n;
   const int K = KC >= 0 ? KC : Krt;
-  cx<R> b0 = a0, b1 = a1;
-  if (act && K > L) {
-    b0 = ld<R>(g.at(K, K - 1));
-    b1 = ld<R>(g.at(K + 1, K - 1));
+  const L2* pb;
+  if constexpr (KC == 0) pb = g.vsh;
+  else if constexpr (KC > 0) pb = g.at(KC, KC - 1);
+  else pb = K > 0 ? g.at(K, K - 1) : g.vsh;
+  const cx<R> b0 = ld<R>(pb);
+  const R b1 = pb[1].x;
+  R nb0;
+  Refl<R> q = larfg2r(b0, b1, nb0);
+  // (a1 = 0 with Im a0 != 0 is an ordinary reflector: t2 = 0, v2 = 0)
+  const uint64_t zero_in = __builtin_amdgcn_ballot_w64(b1 == R(0)) & __builtin_amdgcn_ballot_w64(act);
+  if (__builtin_expect(zero_in != 0, 0)) {
+    if (b1 == R(0) && b0.im == R(0)) {  // zlarfg's identity
+      q.t1 = mk<R>(0, 0);
+      q.v2 = mk<R>(0, 0);
+      q.t2 = R(0);
+      nb0 = b0.re;
+    }
   }
-  cx<R> t1 = mk<R>(0, 0), v2 = mk<R>(0, 0), nb0 = b0;
-  if (act) larfg2(b0, b1, t1, v2, nb0);
-  const R t2 = t1.re * v2.re - t1.im * v2.im;  // Re(t1 * v2)
-  if (act && K > L && g.r == 0) {
-    st<R>(g.at(K, K - 1), nb0);
-    st<R>(g.at(K + 1, K - 1), mk<R>(0, 0));
+  // H(K, K-1) = beta, H(K+1, K-1) = 0; at K = L this overwrites the staged
+  // first column, which lahqr clears after the sweep
+  if (act && K > 0 && g.r == 0) {
+    L2* pk = g.at(K, K > 0 ? K - 1 : 0);
+    pk[0].x = nb0;
+    pk[0].y = R(0);
+    st<R>(pk + 1, mk<R>(0, 0));
   }
   // rows K, K+1 over columns K..n-1 (slot = column)
 #pragma unroll
@@ -517,9 +591,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
         L2* p1 = g.at(K, col);
         L2* p2 = g.at(K + 1, col);
         cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
-        const cx<R> sum = add(cmul(t1, h1), scale(h2, t2));
-        h1 = sub(h1, sum);
-        h2 = sub(h2, mul(sum, v2));
+        refl_left(q, h1, h2);
         st<R>(p1, h1);
         st<R>(p2, h2);
       }
@@ -536,9 +608,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
         L2* p1 = g.at(row, K);
         L2* p2 = g.at(row, K + 1);
         cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
-        const cx<R> sum = add(mul(t1, h1), scale(h2, t2));
-        h1 = sub(h1, sum);
-        h2 = sub(h2, mulc(sum, v2));
+        refl_right(q, h1, h2);
         st<R>(p1, h1);
         st<R>(p2, h2);
       }
@@ -549,11 +619,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
     auto upd = [&](auto kc) {
       constexpr int KK = decltype(kc)::value;
 #pragma unroll
-      for (int p = 0; p < P; ++p) {
-        const cx<R> sum = add(mul(t1, z[p][KK]), scale(z[p][KK + 1], t2));
-        z[p][KK] = sub(z[p][KK], sum);
-        z[p][KK + 1] = sub(z[p][KK + 1], mulc(sum, v2));
-      }
+      for (int p = 0; p < P; ++p) refl_right(q, z[p][KK], z[p][KK + 1]);
     };
     if constexpr (!TZ) {
     } else if constexpr (KC >= 0) {
@@ -567,10 +633,10 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
 
 template <int K, int KEND, bool TZ = true, typename R, int NB, int P, int LAY, int LW>
 __device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], bool ready,
-                                            int L, int I, cx<R> a0, cx<R> a1, int kmin, int kmax) {
+                                            int L, int I, int kmin, int kmax) {
   if constexpr (K < KEND) {
-    if (K >= kmin && K < kmax) sweep_step<K, TZ>(g, z, ready && K >= L && K < I, L, I, a0, a1);
-    sweep_range<K + 1, KEND, TZ>(g, z, ready, L, I, a0, a1, kmin, kmax);
+    if (K >= kmin && K < kmax) sweep_step<K, TZ>(g, z, ready && K >= L && K < I, L, I);
+    sweep_range<K + 1, KEND, TZ>(g, z, ready, L, I, kmin, kmax);
   }
 }
 
@@ -588,6 +654,8 @@ __device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY, LW>& g, c
 // TZ = false: the eigenvalues-only iteration (zlahqr_, Toolbox.hpp:1663-1890):
 // no Schur vectors and H updated inside the active block only; the active
 // block evolves exactly as with TZ = true, so the eigenvalues are the same.
+// Every group needs its own g.vsh[0..1]: a sweep that starts at row 0 stages
+// its first column there.
 // --------------------------------------------------------------------------
 template <typename R, int NB, int P, bool TZ = true, int LAY = LAY_DENSE, int LW = 0>
 __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], int itmax,
@@ -637,6 +705,11 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
   constexpr uint64_t kMask = (LG == 64) ? ~0ull : ((1ull << LG) - 1);
   for (;;) {
     // ---- bookkeeping until every matrix either needs a sweep or is done ----
+    // Every subdiagonal entry k = 1..I is tested at once.  The entries at or
+    // below L are frozen and H(L, L-1) is an exact zero, so the highest
+    // negligible k is what the reference's scan from I down to L+1 finds
+    // (Toolbox.hpp:810-867), and one ballot serves every deflation it shows,
+    // from I downwards, without testing again.
     bool ready = false;
     for (;;) {
       const bool need = !done && !ready;
@@ -646,7 +719,7 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
       for (int p = 0; p < P; ++p) {
         bool neg = false;
         const int k = g.idx(p);
-        if (need && its <= itmax && k > L && k <= I) {
+        if (need && its <= itmax && k > 0 && k <= I) {
           const cx<R> h = ld<R>(g.at(k, k - 1));
           if (cabs1(h) <= smlnum) {
             neg = true;
@@ -678,21 +751,23 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
           its = 0;
           if (I < 0) done = true;
         } else {
-          // the largest k with a negligible H(k, k-1) (the reference's scan
-          // from I down to L+1); L is unchanged if there is none
+          uint64_t neg = 0;  // bit k: H(k, k-1) negligible
 #pragma unroll
-          for (int p = 0; p < P; ++p) {
-            const uint64_t bits = (bal[p] >> base) & kMask;
-            if (bits) L = p * LG + 63 - __clzll(bits);
-          }
-          if (L > 0 && g.r == 0) st<R>(g.at(L, L - 1), mk<R>(0, 0));
-          if (L >= I) {
-            --I;
-            L = 0;
+          for (int p = 0; p < P; ++p) neg |= ((bal[p] >> base) & kMask) << (p * LG);
+          for (;;) {
+            const uint64_t cand = I >= 63 ? neg : (neg & ((2ull << I) - 1));
+            L = cand ? 63 - __clzll(cand) : 0;
+            if (L > 0 && g.r == 0) st<R>(g.at(L, L - 1), mk<R>(0, 0));
+            if (L < I) {
+              ready = true;
+              break;
+            }
+            --I;  // H(I, I) has deflated
             its = 0;
-            if (I < 0) done = true;
-          } else {
-            ready = true;
+            if (I < 0) {
+              done = true;
+              break;
+            }
           }
         }
       }
@@ -701,7 +776,12 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
     if (!__any(ready)) break;
 
     // ---- shift and first column (Toolbox.hpp:896-1015) ----
-    cx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+    // The Wilkinson shift t = H(I,I) - u^2 / (x + y), x = (H(I-1,I-1) -
+    // H(I,I)) / 2, y = +-sqrt(x^2 + u^2), needs u only through u^2 =
+    // H(I-1,I) H(I,I-1) (the reference forms u = sqrt(H(I-1,I)) sqrt(H(I,I-1))
+    // and squares it again): one complex square root, scaled by s.
+    cx<R> a0 = mk<R>(0, 0);
+    R a1 = R(0);
     if (ready) {
       cx<R> t;
       if (its == 10) {
@@ -714,19 +794,16 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
         t.re = s + t.re;
       } else {
         t = ld<R>(g.at(I, I));
-        cx<R> u = mul(csqrt(ld<R>(g.at(I - 1, I))), csqrt(ld<R>(g.at(I, I - 1))));
-        R s = cabs1(u);
-        if (s != R(0)) {
-          cx<R> x = scale(sub(ld<R>(g.at(I - 1, I - 1)), t), R(0.5));
+        const cx<R> u2 = mul(ld<R>(g.at(I - 1, I)), ld<R>(g.at(I, I - 1)));
+        if (u2.re != R(0) || u2.im != R(0)) {
+          const cx<R> x = scale(sub(ld<R>(g.at(I - 1, I - 1)), t), R(0.5));
           const R sx = cabs1(x);
-          s = fmax(s, sx);
+          const R s = fmax(fsqrt(cabs1(u2)), sx);
           const R is = frcp(s);
-          const cx<R> xs = scale(x, is), us = scale(u, is);
-          cx<R> y = scale(csqrt(add(mul(xs, xs), mul(us, us))), s);
-          if (sx > R(0)) {
-            if (x.re * y.re + x.im * y.im < R(0)) y = mk(-y.re, -y.im);
-          }
-          t = sub(t, mul(cdiv(u, add(x, y)), u));
+          const cx<R> xs = scale(x, is);
+          cx<R> ys = csqrt(add(mul(xs, xs), scale(u2, is * is)));  // y / s
+          if (sx > R(0) && xs.re * ys.re + xs.im * ys.im < R(0)) ys = mk(-ys.re, -ys.im);
+          t = sub(t, mul(scale(u2, is), inv(add(xs, ys))));
         }
       }
       const cx<R> h11s = sub(ld<R>(g.at(L, L)), t);
@@ -734,7 +811,15 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
       const R s = cabs1(h11s) + fabs(h21);
       const R is = frcp(s);
       a0 = scale(h11s, is);
-      a1 = mk<R>(h21 * is, 0);
+      a1 = h21 * is;
+      // the first step's reflector input, where sweep_step reads it: column
+      // L-1, rows L and L+1 (H(L, L-1) is a deflated zero; both are restored
+      // after the sweep), or vsh when L = 0
+      if (g.r == 0) {
+        typename Group<R, NB, P, LAY, LW>::L2* sp = L > 0 ? g.at(L, L > 0 ? L - 1 : 0) : g.vsh;
+        st<R>(sp, a0);
+        st<R>(sp + 1, mk<R>(a1, 0));
+      }
     }
 
     // ---- wavefront-uniform range of bulge positions ----
@@ -749,15 +834,20 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
         kmax = iq > kmax ? iq : kmax;
       }
     }
+    wave_sync();
 
     // ---- the sweep (Toolbox.hpp:138-335), K = bulge row ----
     if constexpr (MPW == 1) {
       // one matrix per wavefront: a runtime loop over K, with the Schur
       // vector columns selected by a uniform branch tree
       static_assert(P == 1, "one matrix per wavefront uses one index per lane");
-      for (int K = kmin; K < kmax; ++K) sweep_step<-1, TZ>(g, z, ready && K >= L && K < I, L, I, a0, a1, K);
+      for (int K = kmin; K < kmax; ++K) sweep_step<-1, TZ>(g, z, ready && K >= L && K < I, L, I, K);
     } else {
-      sweep_range<0, NB - 1, TZ>(g, z, ready, L, I, a0, a1, kmin, kmax);
+      sweep_range<0, NB - 1, TZ>(g, z, ready, L, I, kmin, kmax);
+    }
+    if (ready && L > 0 && g.r == 0) {  // the staged first column
+      st<R>(g.at(L, L - 1), mk<R>(0, 0));
+      st<R>(g.at(L + 1, L - 1), mk<R>(0, 0));
     }
 
     // ---- make H(I, I-1) real (Toolbox.hpp:337-367) ----

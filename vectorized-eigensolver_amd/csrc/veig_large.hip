@@ -540,6 +540,67 @@ __device__ __forceinline__ void replay_sweeps_half(const QrShared64 &sh, int h, 
   }
 }
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// The back-transform V = Z X of ztrevc (Toolbox.hpp:1268-1275) for the 32
+// rows row0..row0+31 of Z this wavefront holds as half rows (lane l: row
+// row0 + l % 32, columns 32 (l / 32) .. +31 in zh), on the FP64 matrix
+// cores: v_mfma_f64_16x16x4f64 multiplies a 16 x 4 slice of Z (A operand:
+// lane (c, q) = (l % 16, l / 16) supplies Z(16 I + c, k0 + q)) by a 4 x 16
+// slice of X (B operand: X(k0 + q, 16 J + c), read from the tile, where the
+// solve left y = -x above the diagonal) into a 16 x 16 block of V (lane
+// (c, q), slot s: V(16 I + 4 s + q, 16 J + c)).  The complex product is four
+// real products.  The A operand's entry lives in lane 16 I + c + 32 (k0 / 32)
+// of the wavefront, register k0 % 32 + q: four lane permutes per slice (one
+// per q) and a select.  X is unit upper triangular: column block J needs the
+// row blocks K <= J only.  The result goes straight to the V planes.
+__device__ __forceinline__ void backtransform_mfma(const GBand &g, const cx<double> (&zh)[32], int row0, int lane,
+                                                   int64_t base, const FusedArgs &a) {
+  const int n = g.n;
+  const int c = lane & 15, q = lane >> 4;
+  // one 16 x 16 block of V at a time (runtime loops: Z's 32 half-row entries
+  // take most of the registers, so only one block's accumulators stay live)
+  for (int blk = 0; blk < 8; ++blk) {
+    const int I = blk & 1, J = blk >> 1;
+    if (16 * J >= n) break;  // wavefront-uniform
+    d4 vre = d4{0, 0, 0, 0}, vim = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int K = 0; K < 4; ++K) {
+      if (K > J) break;  // X is upper triangular
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int k = 16 * K + 4 * s + q, j = 16 * J + c;
+        cx<double> x = mk<double>(k == j ? 1.0 : 0.0, 0);
+        if (k < j) {
+          const cx<double> y = ld<double>(g.at(k, j));
+          x = mk<double>(-y.re, -y.im);
+        }
+        const int src = 16 * I + c + 32 * (K >> 1);
+        cx<double> za = mk<double>(0, 0);
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const cx<double> zr = zh[16 * (K & 1) + 4 * s + qq];
+          const double re = __shfl(zr.re, src), im = __shfl(zr.im, src);
+          if (qq == q) za = mk<double>(re, im);
+        }
+        vre = __builtin_amdgcn_mfma_f64_16x16x4f64(za.re, x.re, vre, 0, 0, 0);
+        vre = __builtin_amdgcn_mfma_f64_16x16x4f64(-za.im, x.im, vre, 0, 0, 0);
+        vim = __builtin_amdgcn_mfma_f64_16x16x4f64(za.re, x.im, vim, 0, 0, 0);
+        vim = __builtin_amdgcn_mfma_f64_16x16x4f64(za.im, x.re, vim, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int row = row0 + 16 * I + 4 * s + q, col = 16 * J + c;
+      if (row < n && col < n) {
+        const int64_t idx = base + a.G.el(row, col);
+        a.rV[idx] = vre[s];
+        a.iV[idx] = vim[s];
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k_qr64(FusedArgs a) {
   __shared__ QrShared64 sh;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -616,24 +677,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
     replay_sweeps_half(sh, h, zh);
     __syncthreads();  // the eigenvector factor is ready
-    // V(row, k) = Z(row, k) - sum_{j<k} Z(row, j) y(j, k): each lane sums its
-    // half of the row, the two halves are added through the lane swap, and
-    // the lanes with h = k % 2 write column k
-    for (int k = 0; k < n; ++k) {
-      cx<double> acc = mk<double>(0, 0);
-#pragma unroll
-      for (int c = 0; c < 32; ++c) {
-        const int j = 32 * h + c;
-        if (j < k) acc = add(acc, mul(zh[c], ld<double>(g.at(j, k))));
-        else if (j == k) acc = sub(acc, zh[c]);
-      }
-      acc = add(acc, half_swap(acc));
-      if (row < n && h == (k & 1)) {
-        const int64_t idx = base + a.G.el(row, k);
-        a.rV[idx] = -acc.re;
-        a.iV[idx] = -acc.im;
-      }
-    }
+    backtransform_mfma(g, zh, (w - 1) * 32, lane, base, a);
   }
 }
 

@@ -148,7 +148,7 @@ int veig_zgeev_mixed(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const d
                      const double *iA, double *rE, double *iE, double *rV, double *iV,
                      int32_t *info, int32_t *fallback, double *berr, double tol, int32_t itmax,
                      void *workspace, size_t workspace_bytes, void *stream) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM_FAST));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV);
   VEIG_REQUIRE(iV); VEIG_REQUIRE(workspace);

@@ -61,15 +61,17 @@ __device__ __forceinline__ void companion_rows(const G &g, const EigArgs &a, int
   }
 }
 
-// LDS of a wavefront's MPW matrices in layout LAY (plus the reflector
-// vectors of the Hessenberg reduction for the dense layout)
+// LDS of a wavefront's MPW matrices in layout LAY, and each group's vector:
+// the reflector of the Hessenberg reduction (dense layout), or just the two
+// entries where the QR iteration stages a sweep's first column when it starts
+// at row 0 (every group needs its own)
 template <int NB, int P, int LAY>
 struct EigTiles {
   using G = Group<double, NB, P, LAY>;
   static constexpr int kTile = G::kSize + 1;  // +1: each group starts on another bank
-  static constexpr int kVec = LAY == LAY_DENSE ? NB + 1 : 0;
+  static constexpr int kVec = LAY == LAY_DENSE ? NB + 1 : 2;
   double2 H[G::MPW * kTile];
-  double2 v[G::MPW * kVec + 1];
+  double2 v[G::MPW * kVec];
 };
 
 template <int NB, int P, int SRC, bool TZ>
@@ -172,7 +174,7 @@ extern "C" {
 
 int veig_zlahqr_eig(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *rE, double *iE,
                     double *rH, double *iH, int32_t *info, int32_t itmax, void *stream) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM_FAST));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rH); VEIG_REQUIRE(iH);
   EigArgs a{rH, iH, rH, iH, nullptr, nullptr, rE, iE, info, make_geom(dim, block_cnt, thread_cnt),
@@ -182,7 +184,7 @@ int veig_zlahqr_eig(int64_t dim, int64_t block_cnt, int64_t thread_cnt, double *
 
 int veig_eigvals(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rA, const double *iA,
                  double *rE, double *iE, int32_t *info, int32_t itmax, void *stream) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM_FAST));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rA); VEIG_REQUIRE(iA); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE);
   EigArgs a{rA, iA, nullptr, nullptr, nullptr, nullptr, rE, iE, info, make_geom(dim, block_cnt, thread_cnt),
@@ -193,7 +195,7 @@ int veig_eigvals(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const doubl
 int veig_zlahqr_roots(int64_t dim, int64_t block_cnt, int64_t thread_cnt, const double *rC,
                       const double *iC, double *rE, double *iE, double *rV, double *iV, double *rZ,
                       double *iZ, int32_t *info, int32_t itmax, void *stream) {
-  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM));
+  VEIG_TRY(check_geom(dim, block_cnt, thread_cnt, VEIG_MAX_DIM_FAST));
   if (empty_batch(dim, block_cnt, thread_cnt)) return VEIG_OK;
   VEIG_REQUIRE(rC); VEIG_REQUIRE(iC); VEIG_REQUIRE(rE); VEIG_REQUIRE(iE);
   const bool schur = rV || iV || rZ || iZ;

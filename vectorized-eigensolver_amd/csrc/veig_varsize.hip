@@ -258,7 +258,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(qr_band_wave
 }
 
 constexpr size_t kAlign = 256;
-size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+constexpr size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 size_t varsize_ws_bytes(int64_t nmat) {
   return align_up(sizeof(BinState)) + align_up((size_t)nmat * sizeof(int32_t));

@@ -7,9 +7,9 @@ layout, with the compute running on the gfx950 kernels through libveig.so.
     import python_interface as pi
     pi.my_zgehrd(dim, block_cnt, thread_cnt, rA, iA, rtau, itau)
 
-Layout helpers (my_repMatrix, my_repVector, arrangeMatrices) are host loops in
-the reference (utils.cpp:85-157) and are numpy here; their device versions are
-veig.layout().  The test_* helpers draw their random matrices from an optional
+Layout helpers (my_repMatrix, my_repVector, arrangeMatrices,
+arrangeMatricesAoS) are host loops in the reference (utils.cpp:85-157): here
+numpy on numpy arrays, and the device converters (veig.layout) on GPU tensors.  The test_* helpers draw their random matrices from an optional
 numpy Generator (the reference's are unseeded, python_interface.pyx:98-220)
 and return the same quantities.
 """
@@ -34,28 +34,52 @@ def _p(a: np.ndarray):
 # ---------------------------------------------------------------------------
 # layout helpers (utils.cpp:85-157, python_interface.pyx:31-39, 54-57, 92-96)
 # ---------------------------------------------------------------------------
+def _on_gpu(*arrays) -> bool:
+    """True if every array is a GPU tensor (the device converters run), False
+    if none is (numpy); a mix is refused."""
+    dev = [hasattr(a, "is_cuda") and a.is_cuda for a in arrays]
+    if any(dev) and not all(dev):
+        raise TypeError("give either numpy arrays or GPU tensors, not both")
+    return all(dev)
+
+
 def my_repMatrix(dim, block_cnt, thread_cnt, matrix, out):
-    """repMatrixSoA: copy one column-major dim x dim plane into every slot."""
+    """repMatrixSoA: copy one column-major dim x dim plane into every slot
+    (veig_rep_matrix_soa for GPU tensors)."""
+    if _on_gpu(matrix, out):
+        veig.layout("repm", dim, block_cnt, thread_cnt, matrix, out)
+        return
     m = np.asarray(matrix, dtype=np.float64)[: dim * dim]
     v = out.reshape(block_cnt, dim * dim, thread_cnt)
     v[...] = m[None, :, None]
 
 
 def my_repVector(dim, block_cnt, thread_cnt, vector, out):
-    """repVectorSoA."""
+    """repVectorSoA (veig_rep_vector_soa for GPU tensors)."""
+    if _on_gpu(vector, out):
+        veig.layout("repv", dim, block_cnt, thread_cnt, vector, out)
+        return
     x = np.asarray(vector, dtype=np.float64)[:dim]
     v = out.reshape(block_cnt, dim, thread_cnt)
     v[...] = x[None, :, None]
 
 
 def arrangeMatrices(dim, block_cnt, thread_cnt, inputMat, outputMat):
-    """rearrangeMatrices: AoS (one column-major matrix after another) -> tiled SoA."""
+    """rearrangeMatrices: AoS (one column-major matrix after another) -> tiled
+    SoA (veig_rearrange_matrices for GPU tensors)."""
+    if _on_gpu(inputMat, outputMat):
+        veig.layout("soa", dim, block_cnt, thread_cnt, inputMat, outputMat)
+        return
     src = inputMat.reshape(block_cnt, thread_cnt, dim * dim)
     outputMat.reshape(block_cnt, dim * dim, thread_cnt)[...] = np.transpose(src, (0, 2, 1))
 
 
 def arrangeMatricesAoS(dim, block_cnt, thread_cnt, inputMat, outputMat):
-    """arrangeMatricesAoS (utils.cpp:105): tiled SoA -> AoS."""
+    """arrangeMatricesAoS (utils.cpp:105): tiled SoA -> AoS
+    (veig_arrange_matrices_aos for GPU tensors)."""
+    if _on_gpu(inputMat, outputMat):
+        veig.layout("aos", dim, block_cnt, thread_cnt, inputMat, outputMat)
+        return
     src = inputMat.reshape(block_cnt, dim * dim, thread_cnt)
     outputMat.reshape(block_cnt, thread_cnt, dim * dim)[...] = np.transpose(src, (0, 2, 1))
 

@@ -125,16 +125,67 @@ def _torch():
 
 
 def _ptr(t):
+    """Device address of a GPU tensor (no shape check: callers that know the
+    geometry use _arg)."""
     if t is None:
         return None
-    torch = _torch()
     if not t.is_cuda:
         raise VeigError("device entry points need GPU tensors")
     if not t.is_contiguous():
         raise VeigError("tensors must be contiguous")
-    if t.dtype not in (torch.float64, torch.int32, torch.int64, torch.int8, torch.uint8):
-        raise VeigError(f"unexpected dtype {t.dtype}")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _arg(t, name: str, dtype: str, numel: int, optional: bool = False):
+    """`t` after checking what the kernels will assume of it: dtype (float64
+    planes, int32 info/fallback/dims, int64 offsets, int8 variant table, uint8
+    workspace), at least `numel` elements, contiguous.  An undersized or
+    mistyped tensor would be read or written out of bounds by the kernels, so
+    it is refused here; _call then checks that the tensors are on the GPU."""
+    if t is None:
+        if optional:
+            return None
+        raise VeigError(f"{name}: a tensor is required")
+    torch = _torch()
+    want = getattr(torch, dtype)
+    if t.dtype != want:
+        raise VeigError(f"{name}: dtype {t.dtype}, expected {want}")
+    if t.numel() < numel:
+        raise VeigError(f"{name}: {t.numel()} elements, the batch needs at least {numel}")
+    if not t.is_contiguous():
+        raise VeigError(f"{name}: tensors must be contiguous")
+    return t
+
+
+class _StreamRef:
+    """A stream argument, resolved (torch's current stream if None) by _call
+    once the tensors are known to be on the GPU."""
+
+    def __init__(self, stream):
+        self.stream = stream
+
+
+def _call(f, *args):
+    """f(*args) with every tensor argument (checked by _arg) passed as its
+    device address; all of them must be GPU tensors."""
+    if any(hasattr(a, "data_ptr") and not a.is_cuda for a in args):
+        raise VeigError("device entry points need GPU tensors")
+    conv = []
+    for a in args:
+        if hasattr(a, "data_ptr"):
+            a = ctypes.c_void_p(a.data_ptr())
+        elif isinstance(a, _StreamRef):
+            a = _stream(a.stream)
+        conv.append(a)
+    return f(*conv)
+
+
+def _geom(dim: int, block_cnt: int, thread_cnt: int):
+    """(matrices, matrix-plane elements, vector-plane elements) of a batch."""
+    if dim < 0 or block_cnt < 0 or thread_cnt < 1:
+        raise VeigError("invalid batch geometry (dim < 0, block_cnt < 0 or thread_cnt < 1)")
+    nm = block_cnt * thread_cnt
+    return nm, nm * dim * dim, nm * dim
 
 
 def _stream(stream=None):
@@ -150,25 +201,27 @@ def zgeev(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, info=None
     precision: "fp64" (veig_zgeev) or "fp32" (veig_zgeev_fp32, no refinement).
     Returns (rE, iE, rV, iV)."""
     torch = _torch()
-    nm = block_cnt * thread_cnt
+    nm, na, ne = _geom(dim, block_cnt, thread_cnt)
     dev = rA.device
     if out is None:
-        rE = torch.empty(nm * dim, dtype=torch.float64, device=dev)
+        rE = torch.empty(ne, dtype=torch.float64, device=dev)
         iE = torch.empty_like(rE)
-        rV = torch.empty(nm * dim * dim, dtype=torch.float64, device=dev)
+        rV = torch.empty(na, dtype=torch.float64, device=dev)
         iV = torch.empty_like(rV)
     else:
         rE, iE, rV, iV = out
+    if precision not in ("fp64", "fp32"):
+        raise ValueError(precision)
+    args = (_arg(rA, "rA", "float64", na), _arg(iA, "iA", "float64", na),
+            _arg(rE, "rE", "float64", ne), _arg(iE, "iE", "float64", ne),
+            _arg(rV, "rV", "float64", na), _arg(iV, "iV", "float64", na),
+            _arg(info, "info", "int32", nm, optional=True))
     L = lib()
     if precision == "fp64":
-        st = L.veig_zgeev(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
-                          _ptr(rV), _ptr(iV), _ptr(info), itmax,
-                          VEIG_WRITE_SCHUR if write_schur else 0, _stream(stream))
-    elif precision == "fp32":
-        st = L.veig_zgeev_fp32(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
-                               _ptr(rV), _ptr(iV), _ptr(info), itmax, _stream(stream))
+        st = _call(L.veig_zgeev, dim, block_cnt, thread_cnt, *args, itmax,
+                          VEIG_WRITE_SCHUR if write_schur else 0, _StreamRef(stream))
     else:
-        raise ValueError(precision)
+        st = _call(L.veig_zgeev_fp32, dim, block_cnt, thread_cnt, *args, itmax, _StreamRef(stream))
     check(st, "veig_zgeev")
     return rE, iE, rV, iV
 
@@ -177,14 +230,16 @@ def eigvals(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, info=No
             itmax: int = 30, stream=None):
     """Eigenvalues only of general matrices (veig_eigvals).  Returns (rE, iE)."""
     torch = _torch()
-    nm = block_cnt * thread_cnt
+    nm, na, ne = _geom(dim, block_cnt, thread_cnt)
     if out is None:
-        rE = torch.empty(nm * dim, dtype=torch.float64, device=rA.device)
+        rE = torch.empty(ne, dtype=torch.float64, device=rA.device)
         iE = torch.empty_like(rE)
     else:
         rE, iE = out
-    st = lib().veig_eigvals(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
-                            _ptr(info), itmax, _stream(stream))
+    st = _call(lib().veig_eigvals, dim, block_cnt, thread_cnt, _arg(rA, "rA", "float64", na),
+                            _arg(iA, "iA", "float64", na), _arg(rE, "rE", "float64", ne),
+                            _arg(iE, "iE", "float64", ne), _arg(info, "info", "int32", nm, True),
+                            itmax, _StreamRef(stream))
     check(st, "veig_eigvals")
     return rE, iE
 
@@ -194,14 +249,16 @@ def zlahqr_eig(rH, iH, dim: int, block_cnt: int, thread_cnt: int, out=None, info
     """The reference's eigenvalues-only zlahqr_ on Hessenberg matrices
     (veig_zlahqr_eig; H updated in place).  Returns (rE, iE)."""
     torch = _torch()
-    nm = block_cnt * thread_cnt
+    nm, na, ne = _geom(dim, block_cnt, thread_cnt)
     if out is None:
-        rE = torch.empty(nm * dim, dtype=torch.float64, device=rH.device)
+        rE = torch.empty(ne, dtype=torch.float64, device=rH.device)
         iE = torch.empty_like(rE)
     else:
         rE, iE = out
-    st = lib().veig_zlahqr_eig(dim, block_cnt, thread_cnt, _ptr(rE), _ptr(iE), _ptr(rH), _ptr(iH),
-                               _ptr(info), itmax, _stream(stream))
+    st = _call(lib().veig_zlahqr_eig, dim, block_cnt, thread_cnt, _arg(rE, "rE", "float64", ne),
+                               _arg(iE, "iE", "float64", ne), _arg(rH, "rH", "float64", na),
+                               _arg(iH, "iH", "float64", na), _arg(info, "info", "int32", nm, True),
+                               itmax, _StreamRef(stream))
     check(st, "veig_zlahqr_eig")
     return rE, iE
 
@@ -213,15 +270,18 @@ def roots(rC, iC, dim: int, block_cnt: int, thread_cnt: int, schur: bool = False
     (veig_zlahqr_roots).  Returns (rE, iE) or, with schur, (rE, iE, rT, iT,
     rZ, iZ)."""
     torch = _torch()
-    nm = block_cnt * thread_cnt
+    nm, na, ne = _geom(dim, block_cnt, thread_cnt)
     dev = rC.device
-    rE = torch.empty(nm * dim, dtype=torch.float64, device=dev)
+    rE = torch.empty(ne, dtype=torch.float64, device=dev)
     iE = torch.empty_like(rE)
     mats = [None] * 4
     if schur:
-        mats = [torch.empty(nm * dim * dim, dtype=torch.float64, device=dev) for _ in range(4)]
-    st = lib().veig_zlahqr_roots(dim, block_cnt, thread_cnt, _ptr(rC), _ptr(iC), _ptr(rE), _ptr(iE),
-                                 *[_ptr(m) for m in mats], _ptr(info), itmax, _stream(stream))
+        mats = [torch.empty(na, dtype=torch.float64, device=dev) for _ in range(4)]
+    nc = nm * (dim + 1)
+    st = _call(lib().veig_zlahqr_roots, dim, block_cnt, thread_cnt, _arg(rC, "rC", "float64", nc),
+                                 _arg(iC, "iC", "float64", nc), _ptr(rE), _ptr(iE),
+                                 *[_ptr(m) for m in mats], _arg(info, "info", "int32", nm, True),
+                                 itmax, _StreamRef(stream))
     check(st, "veig_zlahqr_roots")
     return (rE, iE, *mats) if schur else (rE, iE)
 
@@ -240,56 +300,80 @@ def zgeev_mixed(rA, iA, dim: int, block_cnt: int, thread_cnt: int, out=None, inf
     with an FP64 re-solve of the matrices that do not get there.
     Returns (rE, iE, rV, iV)."""
     torch = _torch()
-    nm = block_cnt * thread_cnt
+    nm, na, ne = _geom(dim, block_cnt, thread_cnt)
     dev = rA.device
     if out is None:
-        rE = torch.empty(nm * dim, dtype=torch.float64, device=dev)
+        rE = torch.empty(ne, dtype=torch.float64, device=dev)
         iE = torch.empty_like(rE)
-        rV = torch.empty(nm * dim * dim, dtype=torch.float64, device=dev)
+        rV = torch.empty(na, dtype=torch.float64, device=dev)
         iV = torch.empty_like(rV)
     else:
         rE, iE, rV, iV = out
     if workspace is None:
         workspace = mixed_workspace(dim, block_cnt, thread_cnt, dev)
-    st = lib().veig_zgeev_mixed(dim, block_cnt, thread_cnt, _ptr(rA), _ptr(iA), _ptr(rE), _ptr(iE),
-                                _ptr(rV), _ptr(iV), _ptr(info), _ptr(fallback), _ptr(berr), tol,
-                                itmax, _ptr(workspace), workspace.numel(), _stream(stream))
+    need = int(lib().veig_mixed_workspace_size(dim, block_cnt, thread_cnt))
+    ws = _arg(workspace, "workspace", "uint8", need)
+    st = _call(lib().veig_zgeev_mixed, dim, block_cnt, thread_cnt, _arg(rA, "rA", "float64", na),
+                                _arg(iA, "iA", "float64", na), _arg(rE, "rE", "float64", ne),
+                                _arg(iE, "iE", "float64", ne), _arg(rV, "rV", "float64", na),
+                                _arg(iV, "iV", "float64", na), _arg(info, "info", "int32", nm, True),
+                                _arg(fallback, "fallback", "int32", nm, True),
+                                _arg(berr, "berr", "float64", nm, True), tol, itmax, ws,
+                                workspace.numel() * workspace.element_size(), _StreamRef(stream))
     check(st, "veig_zgeev_mixed")
     return rE, iE, rV, iV
 
 
-def _stage(name, *args, stream=None):
+def _stage(name, dim, bc, tc, *planes, stream=None):
+    """A stage entry (name, geometry, then its float64 planes as
+    (tensor, label, 'm' for an n x n plane or 'v' for a vector plane))."""
+    _, na, ne = _geom(dim, bc, tc)
+    conv = [_arg(t, label, "float64", na if kind == "m" else ne) for t, label, kind in planes]
     f = getattr(lib(), name)
-    conv = [(_ptr(a) if hasattr(a, "data_ptr") else a) for a in args]
-    check(f(*conv, _stream(stream)), name)
+    check(_call(f, dim, bc, tc, *conv, _StreamRef(stream)), name)
 
 
 def zgehrd(dim, bc, tc, rA, iA, rtau, itau, stream=None):
-    _stage("veig_zgehrd", dim, bc, tc, rA, iA, rtau, itau, stream=stream)
+    _stage("veig_zgehrd", dim, bc, tc, (rA, "rA", "m"), (iA, "iA", "m"), (rtau, "rtau", "v"),
+           (itau, "itau", "v"), stream=stream)
 
 
 def zunghr(dim, bc, tc, rA, iA, rtau, itau, stream=None):
-    _stage("veig_zunghr", dim, bc, tc, rA, iA, rtau, itau, stream=stream)
+    _stage("veig_zunghr", dim, bc, tc, (rA, "rA", "m"), (iA, "iA", "m"), (rtau, "rtau", "v"),
+           (itau, "itau", "v"), stream=stream)
 
 
 def hessenberg(dim, bc, tc, rA, iA, rQ, iQ, stream=None):
-    _stage("veig_hessenberg", dim, bc, tc, rA, iA, rQ, iQ, stream=stream)
+    _stage("veig_hessenberg", dim, bc, tc, (rA, "rA", "m"), (iA, "iA", "m"), (rQ, "rQ", "m"),
+           (iQ, "iQ", "m"), stream=stream)
 
 
 def zlahqr(dim, bc, tc, rE, iE, rH, iH, rZ, iZ, info=None, itmax=30, stream=None):
+    nm, na, ne = _geom(dim, bc, tc)
     f = lib().veig_zlahqr
-    check(f(dim, bc, tc, _ptr(rE), _ptr(iE), _ptr(rH), _ptr(iH), _ptr(rZ), _ptr(iZ), _ptr(info),
-            itmax, _stream(stream)), "veig_zlahqr")
+    check(_call(f, dim, bc, tc, _arg(rE, "rE", "float64", ne), _arg(iE, "iE", "float64", ne),
+            _arg(rH, "rH", "float64", na), _arg(iH, "iH", "float64", na),
+            _arg(rZ, "rZ", "float64", na), _arg(iZ, "iZ", "float64", na),
+            _arg(info, "info", "int32", nm, True), itmax, _StreamRef(stream)), "veig_zlahqr")
 
 
 def ztrevc(dim, bc, tc, rT, iT, rR, iR, stream=None):
-    _stage("veig_ztrevc", dim, bc, tc, rT, iT, rR, iR, stream=stream)
+    _stage("veig_ztrevc", dim, bc, tc, (rT, "rT", "m"), (iT, "iT", "m"), (rR, "rR", "m"),
+           (iR, "iR", "m"), stream=stream)
 
 
 def layout(kind: str, dim, bc, tc, src, dst, stream=None):
+    """The device layout converters: "soa" (AoS -> tiled SoA), "aos" (tiled
+    SoA -> AoS), "repm" (one n x n matrix into every slot), "repv" (one
+    vector of n into every slot); one float64 plane each."""
     name = {"soa": "veig_rearrange_matrices", "aos": "veig_arrange_matrices_aos",
             "repm": "veig_rep_matrix_soa", "repv": "veig_rep_vector_soa"}[kind]
-    _stage(name, dim, bc, tc, src, dst, stream=stream)
+    _, na, ne = _geom(dim, bc, tc)
+    n_src = {"soa": na, "aos": na, "repm": dim * dim, "repv": dim}[kind]
+    n_dst = ne if kind == "repv" else na
+    f = getattr(lib(), name)
+    check(_call(f, dim, bc, tc, _arg(src, "src", "float64", n_src), _arg(dst, "dst", "float64", n_dst),
+            _StreamRef(stream)), name)
 
 
 # --------------------------------------------------------------------------
@@ -334,18 +418,23 @@ class VarBatch:
         if self.grid is not None:   # host array, read when the launches are enqueued
             self._grid_host = (ctypes.c_int32 * 5)(*[int(x) for x in self.grid])
             grid = ctypes.cast(self._grid_host, ctypes.c_void_p)
-        return (self.nmat, _ptr(self.dims), _ptr(self.a_off), _ptr(self.e_off), _ptr(self.rA),
-                _ptr(self.iA), _ptr(self.rE), _ptr(self.iE), _ptr(self.rV), _ptr(self.iV),
-                _ptr(self.info), itmax, _ptr(self.variant_table), grid, _ptr(self.ws),
+        n, ta, te = self.nmat, self.total_a, self.total_e
+        return (n, _arg(self.dims, "dims", "int32", n), _arg(self.a_off, "a_off", "int64", n),
+                _arg(self.e_off, "e_off", "int64", n), _arg(self.rA, "rA", "float64", ta),
+                _arg(self.iA, "iA", "float64", ta), _arg(self.rE, "rE", "float64", te),
+                _arg(self.iE, "iE", "float64", te), _arg(self.rV, "rV", "float64", ta),
+                _arg(self.iV, "iV", "float64", ta), _arg(self.info, "info", "int32", n),
+                itmax, _arg(self.variant_table, "variant_table", "int8", 33, optional=True), grid,
+                _arg(self.ws, "workspace", "uint8", int(lib().veig_varsize_workspace_size(n))),
                 self.ws.numel())
 
     def run(self, itmax: int = 30, stream=None):
-        check(lib().veig_zgeev_varsize(*self._args(itmax), _stream(stream)), "veig_zgeev_varsize")
+        check(_call(lib().veig_zgeev_varsize, *self._args(itmax), _StreamRef(stream)), "veig_zgeev_varsize")
 
     def graph(self, itmax: int = 30):
         """An instantiated hipGraph of the whole sequence (binning + solves)."""
         h = ctypes.c_void_p()
-        check(lib().veig_varsize_graph_create(*self._args(itmax), ctypes.byref(h)),
+        check(_call(lib().veig_varsize_graph_create, *self._args(itmax), ctypes.byref(h)),
               "veig_varsize_graph_create")
         return Graph(h)
 
@@ -355,8 +444,9 @@ class VarBatch:
         torch = _torch()
         tab = (ctypes.c_int8 * 33)()
         grid = (ctypes.c_int32 * 5)()
-        args = self._args(itmax)[:12]
-        check(lib().veig_varsize_autotune(*args, tab, grid, _ptr(self.ws), self.ws.numel(), _stream(stream)),
+        full = self._args(itmax)
+        check(_call(lib().veig_varsize_autotune, *full[:12], tab, grid, full[14], self.ws.numel(),
+                                          _StreamRef(stream)),
               "veig_varsize_autotune")
         dev = self.dims.device
         self.variant_table = torch.tensor(list(tab), dtype=torch.int8, device=dev)

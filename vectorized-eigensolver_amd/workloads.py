@@ -11,9 +11,9 @@ with torch (fixed seeds), in the layouts the entry points take.
   c5  65,536 complex128 64x64, re/im iid N(0,1)              tile 1 (one matrix
       after another: a wavefront per matrix reads and writes whole columns)
 
-V is built as U diag(sigma) W^H with random unitary U, W (products of random
-Householder reflections and a random diagonal phase) and sigma log-spaced
-from 1 down to 1/cond; V^{-1} = W diag(1/sigma) U^H.
+V is built as U diag(sigma) W^H with Haar-random unitary U, W (QR of a complex
+Gaussian matrix with the phase fix) and sigma log-spaced from 1 down to
+1/cond; V^{-1} = W diag(1/sigma) U^H.
 """
 from __future__ import annotations
 
@@ -60,25 +60,22 @@ def coeff_soa(N: int, n: int, tile: int, seed: int, device="cuda"):
     return re, im
 
 
-def random_unitary(N: int, n: int, g, device, reflectors: int = 4):
-    """Random unitary matrices: a product of `reflectors` Householder
-    reflections with random directions times a random diagonal phase
-    (batched rank-1 updates, no factorisation)."""
-    ph = torch.rand(N, n, dtype=torch.float64, device=device, generator=g) * (2 * math.pi)
-    Q = torch.diag_embed(torch.polar(torch.ones_like(ph), ph))
-    for _ in range(reflectors):
-        u = torch.complex(torch.randn(N, n, dtype=torch.float64, device=device, generator=g),
-                          torch.randn(N, n, dtype=torch.float64, device=device, generator=g))
-        u = u / torch.linalg.vector_norm(u, dim=1, keepdim=True)
-        Q = Q - 2 * u.unsqueeze(2) * (u.conj().unsqueeze(1) @ Q)
-    return Q
+def haar_unitary(N: int, n: int, g, device):
+    """Haar-distributed unitary matrices: the Q of a complex Gaussian matrix
+    with the phases of R's diagonal moved into Q (Mezzadri, "How to generate
+    random matrices from the classical compact groups", 2007)."""
+    Z = torch.complex(torch.randn(N, n, n, dtype=torch.float64, device=device, generator=g),
+                      torch.randn(N, n, n, dtype=torch.float64, device=device, generator=g))
+    Q, R = torch.linalg.qr(Z)
+    d = torch.diagonal(R, dim1=-2, dim2=-1)
+    return Q * (d / d.abs()).unsqueeze(-2)
 
 
 def conditioned(N: int, n: int, cond, g, device, lam=None):
     """A = V diag(lam) V^{-1} with V = U diag(sigma) W^H, cond(V) = cond
     (scalar or per-matrix tensor); V^{-1} = W diag(1/sigma) U^H is formed
     directly, so no solve is needed."""
-    U, W = random_unitary(N, n, g, device), random_unitary(N, n, g, device)
+    U, W = haar_unitary(N, n, g, device), haar_unitary(N, n, g, device)
     c = torch.as_tensor(cond, dtype=torch.float64, device=device).reshape(-1, 1)
     t = torch.linspace(0, 1, n, dtype=torch.float64, device=device).reshape(1, -1)
     sig = torch.exp(-t * torch.log(c)).expand(N, n)             # 1 .. 1/cond
