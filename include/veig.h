@@ -133,6 +133,19 @@ int veig_zgeev_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off,
                        double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
                        const int8_t *variant_of_n, const int32_t *grid_per_bin, void *workspace,
                        size_t workspace_bytes, void *stream);
+/* veig_zgeev_varsize without a workspace: no counterpart in the reference.
+ * For callers that cannot allocate (code under graph capture, serving loops,
+ * harnesses that forbid hipMalloc): the binning state and permutation live in
+ * a static device array of the library (16 MiB per GPU, held until the
+ * process ends), so the call allocates nothing and enqueues only kernels.
+ * nmat is limited to 4,194,304 (VEIG_ERR_INVALID beyond).  Calls share their
+ * GPU's array: calls that overlap on one GPU (other streams, other threads)
+ * race; calls in order on one stream do not.  The array is looked up on the
+ * first call on each device.  Other arguments as veig_zgeev_varsize. */
+int veig_zgeev_varsize_noalloc(int64_t nmat, const int32_t *dims, const int64_t *a_off,
+                               const int64_t *e_off, const double *rA, const double *iA, double *rE,
+                               double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
+                               const int8_t *variant_of_n, const int32_t *grid_per_bin, void *stream);
 /* Capture the same launch sequence into an instantiated hipGraph (arguments as
  * above; pointers are baked in, array contents are read at each launch). */
 int veig_varsize_graph_create(int64_t nmat, const int32_t *dims, const int64_t *a_off,

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -322,6 +323,38 @@ int enqueue_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, con
   return VEIG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the binning state and the permutation live in a
+// file-scope device array instead of a caller workspace, for callers that
+// cannot allocate (graph capture, serving loops).  Its size caps the batch at
+// kNoallocMaxMat matrices (16 MiB); every GPU the library runs on holds one
+// copy until the process ends.  Host code never uses the array's name as an
+// address: each device's copy is looked up once (hipGetSymbolAddress, with
+// that device current) and that address is what the kernels get.
+constexpr int64_t kNoallocMaxMat = int64_t(1) << 22;
+constexpr size_t kNoallocBytes = align_up(sizeof(BinState)) + align_up(size_t(kNoallocMaxMat) * sizeof(int32_t));
+__device__ __align__(256) char scratch[kNoallocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_addr[kMaxDevices];
+
+int scratch_for_current_device(void **out) {
+  int dev = 0;
+  VEIG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(VEIG_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_addr[dev]) {
+    void *p = nullptr;
+    VEIG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p || reinterpret_cast<uintptr_t>(p) % kAlign != 0)
+      return fail(VEIG_ERR_HIP, "scratch array address missing or misaligned");
+    scratch_addr[dev] = p;
+  }
+  *out = scratch_addr[dev];
+  return VEIG_OK;
+}
+
 }  // namespace
 
 struct veig_graph {
@@ -352,6 +385,25 @@ int veig_zgeev_varsize(int64_t nmat, const int32_t *dims, const int64_t *a_off, 
     return fail(VEIG_ERR_INVALID, "workspace must be 256-byte aligned");
   return enqueue_varsize(nmat, dims, a_off, e_off, rA, iA, rE, iE, rV, iV, info, itmax, variant_of_n,
                          grid_per_bin, workspace, (hipStream_t)stream);
+}
+
+int veig_zgeev_varsize_noalloc(int64_t nmat, const int32_t *dims, const int64_t *a_off,
+                               const int64_t *e_off, const double *rA, const double *iA, double *rE,
+                               double *iE, double *rV, double *iV, int32_t *info, int32_t itmax,
+                               const int8_t *variant_of_n, const int32_t *grid_per_bin, void *stream) {
+  if (nmat < 0) return fail(VEIG_ERR_INVALID, "nmat < 0");
+  if (nmat == 0) return VEIG_OK;
+  if (nmat > kNoallocMaxMat)
+    return fail(VEIG_ERR_INVALID, "nmat " + std::to_string(nmat) + " exceeds the allocation-free entry's " +
+                                      std::to_string(kNoallocMaxMat) + " matrices");
+  VEIG_REQUIRE(dims); VEIG_REQUIRE(a_off); VEIG_REQUIRE(e_off); VEIG_REQUIRE(rA); VEIG_REQUIRE(iA);
+  VEIG_REQUIRE(rE); VEIG_REQUIRE(iE); VEIG_REQUIRE(rV); VEIG_REQUIRE(iV);
+  void *ws = nullptr;
+  VEIG_TRY(scratch_for_current_device(&ws));
+  // k_bin_init clears the binning state (a kernel, not a memset); the
+  // permutation is written in full before it is read
+  return enqueue_varsize(nmat, dims, a_off, e_off, rA, iA, rE, iE, rV, iV, info, itmax, variant_of_n,
+                         grid_per_bin, ws, (hipStream_t)stream);
 }
 
 int veig_varsize_graph_create(int64_t nmat, const int32_t *dims, const int64_t *a_off,

@@ -75,6 +75,8 @@ _SIGS = {
     "veig_varsize_workspace_size": ([_i64], ctypes.c_size_t),
     "veig_zgeev_varsize": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32,
                             _vp, _vp, _vp, ctypes.c_size_t, _vp], _i),
+    "veig_zgeev_varsize_noalloc": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    ctypes.c_int32, _vp, _vp, _vp], _i),
     "veig_varsize_graph_create": ([_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                    ctypes.c_int32, _vp, _vp, _vp, ctypes.c_size_t,
                                    ctypes.POINTER(ctypes.c_void_p)], _i),
@@ -430,6 +432,13 @@ class VarBatch:
 
     def run(self, itmax: int = 30, stream=None):
         check(_call(lib().veig_zgeev_varsize, *self._args(itmax), _StreamRef(stream)), "veig_zgeev_varsize")
+
+    def run_noalloc(self, itmax: int = 30, stream=None):
+        """The same solve through veig_zgeev_varsize_noalloc (the library's
+        static scratch instead of self.ws; at most 4,194,304 matrices)."""
+        args = self._args(itmax)
+        check(_call(lib().veig_zgeev_varsize_noalloc, *args[:14], _StreamRef(stream)),
+              "veig_zgeev_varsize_noalloc")
 
     def graph(self, itmax: int = 30):
         """An instantiated hipGraph of the whole sequence (binning + solves)."""
