@@ -282,19 +282,18 @@ __device__ int lahqr_tile(const Tile &T, Shared &sh, const GMat &Z, int itmax, b
         b0 = H(K, K - 1);
         b1 = H(K + 1, K - 1).re;
       }
-      double nb0;
-      Refl<double> q = larfg2r(b0, b1, nb0);
-      if (b1 == 0.0 && b0.im == 0.0) {  // zlarfg's identity
-        q.t1 = mk<double>(0, 0);
-        q.v2 = mk<double>(0, 0);
-        q.t2 = 0.0;
-        nb0 = b0.re;
+      double nb0, nrm2;
+      Rot<double> q = rot2(b0, b1, nb0, nrm2);
+      if (nrm2 == 0.0) {  // G = I
+        q.c = mk<double>(1, 0);
+        q.s = 0.0;
+        nb0 = 0.0;
       }
       const cx<double> znext = tid < n ? Z.ld(tid, K + 1) : mk<double>(0, 0);
       // rows K, K+1 over columns K..n-1
       for (int c = K + tid; c < n; c += kThreads) {
         cx<double> h1 = H(K, c), h2 = H(K + 1, c);
-        refl_left(q, h1, h2);
+        rot_left(q, h1, h2);
         st<double>(T.at(K, c), h1);
         st<double>(T.at(K + 1, c), h2);
       }
@@ -307,14 +306,14 @@ __device__ int lahqr_tile(const Tile &T, Shared &sh, const GMat &Z, int itmax, b
       const int i3 = (K + 2) < I ? (K + 2) : I;
       for (int r = tid; r <= i3; r += kThreads) {
         cx<double> h1 = H(r, K), h2 = H(r, K + 1);
-        refl_right(q, h1, h2);
+        rot_right(q, h1, h2);
         st<double>(T.at(r, K), h1);
         st<double>(T.at(r, K + 1), h2);
       }
       // Schur vectors: columns K, K+1 of row tid
       if (tid < n) {
         cx<double> z0 = zk1, z1 = znext;
-        refl_right(q, z0, z1);
+        rot_right(q, z0, z1);
         Z.st(tid, K, z0);
         zk1 = z1;
       }

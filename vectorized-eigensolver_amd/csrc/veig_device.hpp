@@ -61,7 +61,8 @@ template <> struct limits<float> {
 // --------------------------------------------------------------------------
 // reciprocal and square root from the hardware approximations (v_rcp_f64,
 // v_rsq_f64) refined by Newton's method (each step doubles the correct bits;
-// two steps reach full double precision from the ~2^-26 seeds), in place of
+// the seeds are good to 2^-24.2 (measured over 2^26 arguments across the
+// exponent range), so it takes two steps to reach full double precision), in place of
 // the IEEE division / sqrt expansions.  Arguments are finite and nonzero
 // where rcp/rsq are used (guarded by the callers); rsqrt-based sqrt maps 0 to 0.
 // --------------------------------------------------------------------------
@@ -193,6 +194,11 @@ __device__ __forceinline__ void uniform_dispatch(int k, F&& f) {
 // offset c(c+5)/2: 166 instead of 272 entries at NB = 16.  Enough for the
 // eigenvalues-only QR iteration, which never leaves that band.
 enum { LAY_DENSE = 0, LAY_BAND = 1 };
+// offset of column c in the banded tile (see Group)
+template <int NB>
+__host__ __device__ constexpr int band_off(int c) {
+  return (c * (c + 5)) >> 1;
+}
 
 // LW: lanes per matrix when wider than NB / P (a slot of capacity NB on a
 // lane group of LW lanes, the lanes past NB / P idle in row operations)
@@ -203,7 +209,7 @@ struct Group {
   static constexpr int LD = NB + 1;
   static constexpr int MPW = 64 / LG;  // matrices per wavefront
   // complex entries of one matrix's tile
-  static constexpr int kSize = LAY == LAY_BAND ? (NB - 1) * (NB + 4) / 2 + NB : NB * LD;
+  static constexpr int kSize = LAY == LAY_BAND ? band_off<NB>(NB - 1) + NB : NB * LD;
   using L2 = typename lds2<R>::type;
   L2* H;     // the tile: H(r,c) at H[c*LD + r], or banded (see above)
   L2* vsh;   // NB complex scratch (reflector vector)
@@ -212,7 +218,7 @@ struct Group {
   int n;     // order of this group's matrix (0 if the slot is empty)
   __device__ __forceinline__ int idx(int p) const { return r + p * LG; }
   __device__ __forceinline__ L2* at(int row, int col) const {
-    if constexpr (LAY == LAY_BAND) return H + ((col * (col + 5)) >> 1) + row;
+    if constexpr (LAY == LAY_BAND) return H + band_off<NB>(col) + row;
     else return H + col * LD + row;
   }
 };
@@ -481,68 +487,65 @@ __device__ __forceinline__ void larfg2(cx<R> a0, cx<R> a1, cx<R>& t1, cx<R>& v2,
   beta_out = mk<R>(-beta, 0);
 }
 
-// A bulge-chase reflector G = I - t1 [1; v2][1; v2]^H, with t2 = t1 v2,
-// which the chase keeps real.
+// The unitary of one bulge-chase step.  The reference chases the bulge
+// with zlarfg's 2x2 reflectors (Toolbox.hpp:138-335); a plane rotation
+// G = [c, -s; s, conj(c)] with c = a0 / r, s = a1 / r, r = |[a0; a1]|, maps
+// [a0; a1] to G^H [a0; a1] = [r; 0] as the reflector maps it to [beta; 0],
+// beta = -copysign(r, Re a0).  Both start every step from [a0; a1] with a1
+// real -- the first step's a1 is the scaled subdiagonal entry, every later one
+// the bulge s H(K+2, K+1) of a real subdiagonal entry (Toolbox.hpp:172-174)
+// -- and both leave H(K, K-1) real, so the two chases differ by a similarity
+// with a diagonal of signs: the same diagonal, subdiagonal and entry moduli,
+// the same deflations and shifts, Schur vectors and eigenvectors equal up to
+// the sign of each column.  The rotation needs one reciprocal square root
+// (the reflector one more reciprocal) and applies with the same 12 flops.
 template <typename R>
-struct Refl {
-  cx<R> t1, v2;
-  R t2;
+struct Rot {
+  cx<R> c;
+  R s;
 };
 
-// The reflector of zlarfg with n = 2 (LAPACK sign, Toolbox.hpp:183-242) for
-// [a0; a1] with a1 real: the first step's a1 is the scaled subdiagonal entry,
-// and every later one is the bulge -t2 H(K+1, K) of a real subdiagonal entry
-// (the reference's own remark, Toolbox.hpp:172-174).  With
-// beta = copysign(|[a0; a1]|, Re a0):
-//   t1 = (a0 + beta) / beta,  t2 = t1 v2 = a1 / beta,
-//   v2 = a1 / (a0 + beta) = t2 conj(t1) / |t1|^2,  |t1|^2 = 2 Re t1 - t2^2
-// (the last from G being unitary; it lies in [1, 4], so the subtraction does
-// not cancel): one reciprocal square root and one reciprocal.  beta_out =
-// -beta is the new a0.  The zero vector (a1 = 0 = Im a0), where zlarfg
-// returns the identity, is left to the caller.
+// The rotation for [a0; a1], a1 real, r (the new H(K, K-1)) and r^2; where
+// r^2 = 0 (the zero vector, or one below the underflow threshold) 1/r is
+// infinite and the caller takes G = I, r = 0 instead
 template <typename R>
-__device__ __forceinline__ Refl<R> larfg2r(cx<R> a0, R a1, R& beta_out) {
-  const R nrm2 = fma(a0.re, a0.re, fma(a0.im, a0.im, a1 * a1));
-  const R ib = __builtin_copysign(frsq(nrm2), a0.re);  // 1/beta
-  beta_out = -(nrm2 * ib);
-  Refl<R> q;
-  q.t1 = mk(fma(a0.re, ib, R(1)), a0.im * ib);
-  q.t2 = a1 * ib;
-  const R s = q.t2 * frcp(fma(-q.t2, q.t2, q.t1.re + q.t1.re));
-  q.v2 = mk(q.t1.re * s, -q.t1.im * s);
-  return q;
+__device__ __forceinline__ Rot<R> rot2(cx<R> a0, R a1, R& r_out, R& nrm2) {
+  nrm2 = fma(a0.re, a0.re, fma(a0.im, a0.im, a1 * a1));
+  const R ir = frsq(nrm2);
+  r_out = nrm2 * ir;
+  return Rot<R>{mk(a0.re * ir, a0.im * ir), a1 * ir};
 }
 
-// [h1; h2] <- G^H [h1; h2]: the reflector from the left (rows K, K+1);
-// s = conj(t1) h1 + t2 h2, h1 -= s, h2 -= s v2
+// [h1; h2] <- G^H [h1; h2] (rows K, K+1): h1 = conj(c) h1 + s h2,
+// h2 = c h2 - s h1
 template <typename R>
-__device__ __forceinline__ void refl_left(const Refl<R>& q, cx<R>& h1, cx<R>& h2) {
-  const R sr = fma(q.t1.re, h1.re, fma(q.t1.im, h1.im, q.t2 * h2.re));
-  const R si = fma(q.t1.re, h1.im, fma(-q.t1.im, h1.re, q.t2 * h2.im));
-  h1.re -= sr;
-  h1.im -= si;
-  h2.re = fma(-sr, q.v2.re, fma(si, q.v2.im, h2.re));
-  h2.im = fma(-sr, q.v2.im, fma(-si, q.v2.re, h2.im));
+__device__ __forceinline__ void rot_left(const Rot<R>& q, cx<R>& h1, cx<R>& h2) {
+  const R r1 = fma(q.c.re, h1.re, fma(q.c.im, h1.im, q.s * h2.re));
+  const R i1 = fma(q.c.re, h1.im, fma(-q.c.im, h1.re, q.s * h2.im));
+  const R r2 = fma(q.c.re, h2.re, fma(-q.c.im, h2.im, -q.s * h1.re));
+  const R i2 = fma(q.c.re, h2.im, fma(q.c.im, h2.re, -q.s * h1.im));
+  h1 = mk(r1, i1);
+  h2 = mk(r2, i2);
 }
 
-// [h1, h2] <- [h1, h2] G: the reflector from the right (columns K, K+1 of H
-// and of the Schur vectors); s = t1 h1 + t2 h2, h1 -= s, h2 -= s conj(v2)
+// [h1, h2] <- [h1, h2] G (columns K, K+1 of H and of the Schur vectors):
+// h1 = c h1 + s h2, h2 = conj(c) h2 - s h1
 template <typename R>
-__device__ __forceinline__ void refl_right(const Refl<R>& q, cx<R>& h1, cx<R>& h2) {
-  const R sr = fma(q.t1.re, h1.re, fma(-q.t1.im, h1.im, q.t2 * h2.re));
-  const R si = fma(q.t1.re, h1.im, fma(q.t1.im, h1.re, q.t2 * h2.im));
-  h1.re -= sr;
-  h1.im -= si;
-  h2.re = fma(-sr, q.v2.re, fma(-si, q.v2.im, h2.re));
-  h2.im = fma(-si, q.v2.re, fma(sr, q.v2.im, h2.im));
+__device__ __forceinline__ void rot_right(const Rot<R>& q, cx<R>& h1, cx<R>& h2) {
+  const R r1 = fma(q.c.re, h1.re, fma(-q.c.im, h1.im, q.s * h2.re));
+  const R i1 = fma(q.c.re, h1.im, fma(q.c.im, h1.re, q.s * h2.im));
+  const R r2 = fma(q.c.re, h2.re, fma(q.c.im, h2.im, -q.s * h1.re));
+  const R i2 = fma(q.c.re, h2.im, fma(-q.c.im, h2.re, -q.s * h1.im));
+  h1 = mk(r1, i1);
+  h2 = mk(r2, i2);
 }
 
-// Apply one bulge-chase reflector at row K to this lane's slots: rows K, K+1
+// Apply one bulge-chase rotation at row K to this lane's slots: rows K, K+1
 // of H over the lane's columns >= K, columns K, K+1 of H over the lane's rows
 // <= i3, and columns K, K+1 of the Schur vectors.  KC >= 0 is K as a
 // compile-time constant (unrolled sweeps); KC < 0 takes the wavefront-uniform
 // runtime Krt and selects the register columns by a uniform branch tree.
-// The reflector's input is column K-1, rows K and K+1; at the first step of
+// The rotation's input is column K-1, rows K and K+1; at the first step of
 // a matrix's sweep (K = L) the caller has staged the shifted first column
 // (a0; a1 real) there -- in vsh[0..1] when L = 0 -- so that every step reads
 // its input from a fixed place, without a select or a branch.
@@ -556,25 +559,29 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
   constexpr int LG = Group<R, NB, P, LAY, LW>::LG;
   const int n = g.n;
   const int K = KC >= 0 ? KC : Krt;
+  constexpr int MPW = Group<R, NB, P, LAY, LW>::MPW;
+  // with several matrices per wavefront, groups outside their sweep at this
+  // K read the input (1, 0; 0) that lahqr keeps in vsh[2..3], whose rotation
+  // is the identity exactly: the Schur-vector update then needs no branch
+  constexpr bool kIdle = TZ && MPW > 1;
   const L2* pb;
   if constexpr (KC == 0) pb = g.vsh;
   else if constexpr (KC > 0) pb = g.at(KC, KC - 1);
   else pb = K > 0 ? g.at(K, K - 1) : g.vsh;
+  if constexpr (kIdle) pb = act ? pb : g.vsh + 2;
   const cx<R> b0 = ld<R>(pb);
   const R b1 = pb[1].x;
-  R nb0;
-  Refl<R> q = larfg2r(b0, b1, nb0);
-  // (a1 = 0 with Im a0 != 0 is an ordinary reflector: t2 = 0, v2 = 0)
-  const uint64_t zero_in = __builtin_amdgcn_ballot_w64(b1 == R(0)) & __builtin_amdgcn_ballot_w64(act);
-  if (__builtin_expect(zero_in != 0, 0)) {
-    if (b1 == R(0) && b0.im == R(0)) {  // zlarfg's identity
-      q.t1 = mk<R>(0, 0);
-      q.v2 = mk<R>(0, 0);
-      q.t2 = R(0);
-      nb0 = b0.re;
+  R nb0, nrm2;
+  Rot<R> q = rot2(b0, b1, nb0, nrm2);
+  const bool zero_in = nrm2 == R(0);
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(zero_in) != 0, 0)) {
+    if (zero_in) {
+      q.c = mk<R>(1, 0);
+      q.s = R(0);
+      nb0 = R(0);
     }
   }
-  // H(K, K-1) = beta, H(K+1, K-1) = 0; at K = L this overwrites the staged
+  // H(K, K-1) = r, H(K+1, K-1) = 0; at K = L this overwrites the staged
   // first column, which lahqr clears after the sweep
   if (act && K > 0 && g.r == 0) {
     L2* pk = g.at(K, K > 0 ? K - 1 : 0);
@@ -591,7 +598,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
         L2* p1 = g.at(K, col);
         L2* p2 = g.at(K + 1, col);
         cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
-        refl_left(q, h1, h2);
+        rot_left(q, h1, h2);
         st<R>(p1, h1);
         st<R>(p2, h2);
       }
@@ -608,18 +615,18 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
         L2* p1 = g.at(row, K);
         L2* p2 = g.at(row, K + 1);
         cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
-        refl_right(q, h1, h2);
+        rot_right(q, h1, h2);
         st<R>(p1, h1);
         st<R>(p2, h2);
       }
     }
   }
   // Schur vectors: columns K, K+1 of each owned row
-  if (TZ && act) {
+  if (TZ && (kIdle || act)) {
     auto upd = [&](auto kc) {
       constexpr int KK = decltype(kc)::value;
 #pragma unroll
-      for (int p = 0; p < P; ++p) refl_right(q, z[p][KK], z[p][KK + 1]);
+      for (int p = 0; p < P; ++p) rot_right(q, z[p][KK], z[p][KK + 1]);
     };
     if constexpr (!TZ) {
     } else if constexpr (KC >= 0) {
@@ -654,8 +661,9 @@ __device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY, LW>& g, c
 // TZ = false: the eigenvalues-only iteration (zlahqr_, Toolbox.hpp:1663-1890):
 // no Schur vectors and H updated inside the active block only; the active
 // block evolves exactly as with TZ = true, so the eigenvalues are the same.
-// Every group needs its own g.vsh[0..1]: a sweep that starts at row 0 stages
-// its first column there.
+// Every group needs its own g.vsh[0..1] (TZ with several matrices per
+// wavefront: g.vsh[0..3]): a sweep that starts at row 0 stages its first
+// column there.
 // --------------------------------------------------------------------------
 template <typename R, int NB, int P, bool TZ = true, int LAY = LAY_DENSE, int LW = 0>
 __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], int itmax,
@@ -699,6 +707,11 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
     }
   }
 
+  // the identity input of the groups a sweep step passes by (sweep_step)
+  if (TZ && MPW > 1 && g.r == 0) {
+    st<R>(g.vsh + 2, mk<R>(1, 0));
+    st<R>(g.vsh + 3, mk<R>(0, 0));
+  }
   int I = n - 1, L = 0, its = 0, nfail = 0;
   bool done = (n <= 0);
   const int base = g.mi * LG;
