@@ -138,6 +138,31 @@ template <typename R> __device__ __forceinline__ cx<R> csqrt(cx<R> a) {
   return o;
 }
 
+// frexp's exponent (x = m 2^e, 1/2 <= |m| < 1; 0 for x = 0) and ldexp: exact
+// power-of-two scaling
+__device__ __forceinline__ int fexp(double x) { return __builtin_amdgcn_frexp_exp(x); }
+__device__ __forceinline__ int fexp(float x) { return __builtin_amdgcn_frexp_expf(x); }
+__device__ __forceinline__ double fldexp(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+__device__ __forceinline__ float fldexp(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+template <typename R> __device__ __forceinline__ cx<R> cldexp(cx<R> a, int e) {
+  return {fldexp(a.re, e), fldexp(a.im, e)};
+}
+
+// principal square root of an a with |a| of order one (the scaled Wilkinson
+// discriminant): t = sqrt((|a| + |Re a|) / 2) = q rsq(q) and |Im a| / (2 t) =
+// |Im a| rsq(q) / 2, two reciprocal square roots in place of two square roots
+// and a reciprocal; 0 for a = 0
+template <typename R> __device__ __forceinline__ cx<R> csqrt_unit(cx<R> a) {
+  const R m2 = abs2(a);
+  const R m = m2 > R(0) ? m2 * frsq(m2) : R(0);
+  const R q = R(0.5) * (m + fabs(a.re));
+  const R rq = q > R(0) ? frsq(q) : R(0);
+  const R zr = q * rq, zi = fabs(a.im) * (R(0.5) * rq);
+  cx<R> o = (a.re < R(0)) ? mk(zi, zr) : mk(zr, zi);
+  if (a.im < R(0)) o.im = -o.im;
+  return o;
+}
+
 template <typename R>
 __device__ __forceinline__ cx<R> ld(const typename lds2<R>::type* p) {
   const typename lds2<R>::type v = *p;
@@ -517,13 +542,16 @@ __device__ __forceinline__ Rot<R> rot2(cx<R> a0, R a1, R& r_out, R& nrm2) {
 }
 
 // [h1; h2] <- G^H [h1; h2] (rows K, K+1): h1 = conj(c) h1 + s h2,
-// h2 = c h2 - s h1
+// h2 = c h2 - s h1.  Each part is a product and two fused multiply-adds;
+// the minus signs sit on a product or on the addend of the last one, where
+// the instruction carries them for free (an accumulating multiply-add would
+// need the negated factor in a register of its own).
 template <typename R>
 __device__ __forceinline__ void rot_left(const Rot<R>& q, cx<R>& h1, cx<R>& h2) {
   const R r1 = fma(q.c.re, h1.re, fma(q.c.im, h1.im, q.s * h2.re));
-  const R i1 = fma(q.c.re, h1.im, fma(-q.c.im, h1.re, q.s * h2.im));
-  const R r2 = fma(q.c.re, h2.re, fma(-q.c.im, h2.im, -q.s * h1.re));
-  const R i2 = fma(q.c.re, h2.im, fma(q.c.im, h2.re, -q.s * h1.im));
+  const R i1 = fma(q.c.re, h1.im, fma(q.s, h2.im, -(q.c.im * h1.re)));
+  const R r2 = fma(q.c.re, h2.re, -fma(q.c.im, h2.im, q.s * h1.re));
+  const R i2 = fma(q.c.re, h2.im, fma(q.c.im, h2.re, -(q.s * h1.im)));
   h1 = mk(r1, i1);
   h2 = mk(r2, i2);
 }
@@ -532,10 +560,10 @@ __device__ __forceinline__ void rot_left(const Rot<R>& q, cx<R>& h1, cx<R>& h2) 
 // h1 = c h1 + s h2, h2 = conj(c) h2 - s h1
 template <typename R>
 __device__ __forceinline__ void rot_right(const Rot<R>& q, cx<R>& h1, cx<R>& h2) {
-  const R r1 = fma(q.c.re, h1.re, fma(-q.c.im, h1.im, q.s * h2.re));
+  const R r1 = fma(q.c.re, h1.re, fma(q.s, h2.re, -(q.c.im * h1.im)));
   const R i1 = fma(q.c.re, h1.im, fma(q.c.im, h1.re, q.s * h2.im));
-  const R r2 = fma(q.c.re, h2.re, fma(q.c.im, h2.im, -q.s * h1.re));
-  const R i2 = fma(q.c.re, h2.im, fma(-q.c.im, h2.re, -q.s * h1.im));
+  const R r2 = fma(q.c.re, h2.re, fma(q.c.im, h2.im, -(q.s * h1.re)));
+  const R i2 = fma(q.c.re, h2.im, -fma(q.c.im, h2.re, q.s * h1.im));
   h1 = mk(r1, i1);
   h2 = mk(r2, i2);
 }
@@ -589,12 +617,14 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
     pk[0].y = R(0);
     st<R>(pk + 1, mk<R>(0, 0));
   }
-  // rows K, K+1 over columns K..n-1 (slot = column)
+  // rows K, K+1 over columns K..n-1 (slot = column); groups outside their
+  // sweep apply their identity rotation too, where it rewrites what they read
+  const bool upd = kIdle || act;
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     if ((p + 1) * LG - 1 >= K) {
       const int col = g.idx(p);
-      if (act && col >= K && col < n && (TZ || col <= I)) {
+      if (upd && col >= K && col < n && (TZ || col <= I)) {
         L2* p1 = g.at(K, col);
         L2* p2 = g.at(K + 1, col);
         cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
@@ -606,12 +636,11 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
   }
   wave_sync();
   // columns K, K+1 over rows 0..min(K+2, I) (slot = row)
-  const int i3 = (K + 2) < I ? (K + 2) : I;
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     if (p * LG <= K + 2) {
       const int row = g.idx(p);
-      if (act && row <= i3 && (TZ || row >= L)) {
+      if (upd && row <= K + 2 && row <= I && (TZ || row >= L)) {
         L2* p1 = g.at(row, K);
         L2* p2 = g.at(row, K + 1);
         cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
@@ -622,17 +651,17 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
     }
   }
   // Schur vectors: columns K, K+1 of each owned row
-  if (TZ && (kIdle || act)) {
-    auto upd = [&](auto kc) {
+  if (TZ && upd) {
+    auto zupd = [&](auto kc) {
       constexpr int KK = decltype(kc)::value;
 #pragma unroll
       for (int p = 0; p < P; ++p) rot_right(q, z[p][KK], z[p][KK + 1]);
     };
     if constexpr (!TZ) {
     } else if constexpr (KC >= 0) {
-      upd(std::integral_constant<int, KC>{});
+      zupd(std::integral_constant<int, KC>{});
     } else {
-      uniform_dispatch<0, NB - 1>(K, upd);
+      uniform_dispatch<0, NB - 1>(K, zupd);
     }
   }
   wave_sync();
@@ -748,8 +777,8 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
               const R ab = fmax(cabs1(h), cabs1(hup)), ba = fmin(cabs1(h), cabs1(hup));
               const R d3 = cabs1(sub(hkk1, hkk));
               const R aa = fmax(cabs1(hkk), d3), bb = fmin(cabs1(hkk), d3);
-              const R s = aa + ab;
-              neg = ba * (ab / s) <= fmax(smlnum, ulp * (bb * (aa / s)));
+              const R is = frcp(aa + ab);  // aa + ab >= cabs1(h) > 0
+              neg = ba * (ab * is) <= fmax(smlnum, ulp * (bb * (aa * is)));
             }
           }
         }
@@ -792,7 +821,11 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
     // The Wilkinson shift t = H(I,I) - u^2 / (x + y), x = (H(I-1,I-1) -
     // H(I,I)) / 2, y = +-sqrt(x^2 + u^2), needs u only through u^2 =
     // H(I-1,I) H(I,I-1) (the reference forms u = sqrt(H(I-1,I)) sqrt(H(I,I-1))
-    // and squares it again): one complex square root, scaled by s.
+    // and squares it again): one complex square root.  The reference scales
+    // x and u by s = max(|u|, |x|) (cabs1); the power of two 2^E >= s used
+    // here scales exactly and needs no square root or reciprocal, and so
+    // does 2^E1 for the first column (the reference divides it by its cabs1
+    // norm; the rotation normalises it anyway).
     cx<R> a0 = mk<R>(0, 0);
     R a1 = R(0);
     if (ready) {
@@ -811,20 +844,20 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
         if (u2.re != R(0) || u2.im != R(0)) {
           const cx<R> x = scale(sub(ld<R>(g.at(I - 1, I - 1)), t), R(0.5));
           const R sx = cabs1(x);
-          const R s = fmax(fsqrt(cabs1(u2)), sx);
-          const R is = frcp(s);
-          const cx<R> xs = scale(x, is);
-          cx<R> ys = csqrt(add(mul(xs, xs), scale(u2, is * is)));  // y / s
+          // 2^E > max(|u|, |x|): |x / 2^E| < 1 and |u^2 / 2^2E| <= 1
+          const int eu = fexp(cabs1(u2)), ex = sx > R(0) ? fexp(sx) : -4096;
+          const int E = max((eu + 1) >> 1, ex);
+          const cx<R> xs = cldexp(x, -E);
+          cx<R> ys = csqrt_unit(add(mul(xs, xs), cldexp(u2, -2 * E)));  // y / 2^E
           if (sx > R(0) && xs.re * ys.re + xs.im * ys.im < R(0)) ys = mk(-ys.re, -ys.im);
-          t = sub(t, mul(scale(u2, is), inv(add(xs, ys))));
+          t = sub(t, mul(cldexp(u2, -E), inv(add(xs, ys))));
         }
       }
       const cx<R> h11s = sub(ld<R>(g.at(L, L)), t);
       const R h21 = ld<R>(g.at(L + 1, L)).re;
-      const R s = cabs1(h11s) + fabs(h21);
-      const R is = frcp(s);
-      a0 = scale(h11s, is);
-      a1 = h21 * is;
+      const int E1 = fexp(cabs1(h11s) + fabs(h21));
+      a0 = cldexp(h11s, -E1);
+      a1 = fldexp(h21, -E1);
       // the first step's reflector input, where sweep_step reads it: column
       // L-1, rows L and L+1 (H(L, L-1) is a deflated zero; both are restored
       // after the sweep), or vsh when L = 0
