@@ -33,6 +33,7 @@
 #include <stdint.h>
 #include <float.h>
 #include <type_traits>
+#include <utility>
 
 namespace veig {
 
@@ -205,6 +206,16 @@ __device__ __forceinline__ void uniform_dispatch(int k, F&& f) {
   }
 }
 
+// f(integral_constant<int, I>) for I = 0 .. N-1 in order, unrolled
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // --------------------------------------------------------------------------
 // per-group context
 // --------------------------------------------------------------------------
@@ -370,24 +381,25 @@ __device__ __forceinline__ void gehrd_q(const Group<R, NB, P>& g, cx<R> (&z)[P][
 }
 
 // The Hessenberg reduction of gehrd_q (zgehrd with zlarfg's LAPACK sign,
-// Toolbox.hpp:1315-1469) for one index per lane, leaving the compact form:
-// v(i+2:n) below the subdiagonal, tau(i) in tau_out of lane i.  Each update
-// takes the lane's row (right update) or column (left update) and the
-// reflector into registers a chunk of up to 16 entries at a time -- loads
-// the hardware overlaps -- instead of a runtime loop whose every iteration
-// waits on its own LDS round trip; entries at or left of column i are
-// loaded and left alone, chunks wholly there are skipped.  With one chunk
-// (NB <= 16) the update reuses the entries the product loaded.
+// Toolbox.hpp:1315-1469) for one index per lane (NB <= 16), leaving the
+// compact form: v(i+2:n) below the subdiagonal, tau(i) in tau_out of lane i.
+// The reflectors are unrolled, so each update touches only the columns
+// (right update) or rows (left update) i+1..NB-1 that reflector i changes --
+// half the loads and flops of full-width updates -- and takes them into
+// registers at once, loads the hardware overlaps.  The tile is zero outside
+// the n x n matrix and v is zero past n, so entries past n are rewritten
+// unchanged rather than masked.
 template <int NB, int LW = 0>
 __device__ __forceinline__ void gehrd_compact(const Group<double, NB, 1, LAY_DENSE, LW> &g, int nmax,
                                               cx<double> &tau_out) {
   using R = double;
   constexpr int LG = Group<R, NB, 1, LAY_DENSE, LW>::LG;
-  constexpr int CH = NB < 16 ? NB : 16;  // entries per chunk
-  constexpr int NCH = NB / CH;
+  static_assert(NB <= 16 && LG == NB, "one index per lane, at most 16");
   const int n = g.n, r = g.r;
   const bool in = r < n;
-  for (int i = 0; i + 1 < nmax; ++i) {
+  static_for<NB - 1>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    if (i + 1 >= nmax) return;
     // ---- reflector annihilating H(i+2:n, i) ----
     const cx<R> hri = in ? ld<R>(g.at(r, i)) : mk<R>(0, 0);
     const R xnorm2 = group_sum<LG, R>(r >= i + 2 && in ? abs2(hri) : R(0));
@@ -407,85 +419,40 @@ __device__ __forceinline__ void gehrd_compact(const Group<double, NB, 1, LAY_DEN
       if (r == i + 1) vr = mk<R>(1, 0);
       else if (r >= i + 2 && in) vr = mul(hri, scal);
     }
-    if (r < NB) st<R>(g.vsh + r, vr);
+    st<R>(g.vsh + r, vr);
     if (live && in && r >= i + 1) st<R>(g.at(r, i), r == i + 1 ? mk<R>(beta, 0) : vr);
     if (r == i) tau_out = tau;
     wave_sync();
-    // ---- H <- H (I - tau v v^H): row r over columns i+1..n-1 ----
+    constexpr int M = NB - 1 - i;  // columns / rows i+1..NB-1
+    cx<R> h[M], v[M];
+    // ---- H <- H (I - tau v v^H): row r over columns i+1..NB-1 ----
     if (live) {
-      cx<R> h[CH], v[CH];
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        h[j] = ld<R>(g.at(r, i + 1 + j));
+        v[j] = ld<R>(g.vsh + i + 1 + j);
+      }
       cx<R> w = mk<R>(0, 0);
 #pragma unroll
-      for (int q = 0; q < NCH; ++q) {
-        if (CH * q + CH - 1 > i) {
-#pragma unroll
-          for (int j = 0; j < CH; ++j) {
-            h[j] = ld<R>(g.at(r, CH * q + j));
-            v[j] = ld<R>(g.vsh + CH * q + j);  // zero at c <= i and beyond n
-          }
-#pragma unroll
-          for (int j = 0; j < CH; ++j)
-            if (CH * q + j > i) w = add(w, mul(h[j], v[j]));
-        }
-      }
+      for (int j = 0; j < M; ++j) w = add(w, mul(h[j], v[j]));
       w = mul(tau, w);
 #pragma unroll
-      for (int q = 0; q < NCH; ++q) {
-        if (CH * q + CH - 1 > i) {
-          if constexpr (NCH > 1) {
-#pragma unroll
-            for (int j = 0; j < CH; ++j) {
-              h[j] = ld<R>(g.at(r, CH * q + j));
-              v[j] = ld<R>(g.vsh + CH * q + j);
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < CH; ++j) {
-            const int c = CH * q + j;
-            if (c > i && c < n && in) st<R>(g.at(r, c), sub(h[j], mulc(w, v[j])));
-          }
-        }
-      }
+      for (int j = 0; j < M; ++j) st<R>(g.at(r, i + 1 + j), sub(h[j], mulc(w, v[j])));
     }
     wave_sync();
-    // ---- H <- (I - conj(tau) v v^H) H: column r over rows i+1..n-1 ----
-    if (live) {
-      cx<R> h[CH], v[CH];
+    // ---- H <- (I - conj(tau) v v^H) H: column r > i over rows i+1..NB-1 ----
+    if (live && r > i) {
+#pragma unroll
+      for (int j = 0; j < M; ++j) h[j] = ld<R>(g.at(i + 1 + j, r));
       cx<R> y = mk<R>(0, 0);
 #pragma unroll
-      for (int q = 0; q < NCH; ++q) {
-        if (CH * q + CH - 1 > i) {
-#pragma unroll
-          for (int j = 0; j < CH; ++j) {
-            h[j] = ld<R>(g.at(CH * q + j, r));
-            v[j] = ld<R>(g.vsh + CH * q + j);
-          }
-#pragma unroll
-          for (int j = 0; j < CH; ++j)
-            if (CH * q + j > i) y = add(y, cmul(v[j], h[j]));
-        }
-      }
+      for (int j = 0; j < M; ++j) y = add(y, cmul(v[j], h[j]));
       y = cmul(tau, y);  // conj(tau) * y
 #pragma unroll
-      for (int q = 0; q < NCH; ++q) {
-        if (CH * q + CH - 1 > i) {
-          if constexpr (NCH > 1) {
-#pragma unroll
-            for (int j = 0; j < CH; ++j) {
-              h[j] = ld<R>(g.at(CH * q + j, r));
-              v[j] = ld<R>(g.vsh + CH * q + j);
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < CH; ++j) {
-            const int k = CH * q + j;
-            if (k > i && k < n && r > i && in) st<R>(g.at(k, r), sub(h[j], mul(v[j], y)));
-          }
-        }
-      }
+      for (int j = 0; j < M; ++j) st<R>(g.at(i + 1 + j, r), sub(h[j], mul(v[j], y)));
     }
     wave_sync();
-  }
+  });
 }
 
 // --------------------------------------------------------------------------
@@ -597,6 +564,17 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
   else if constexpr (KC > 0) pb = g.at(KC, KC - 1);
   else pb = K > 0 ? g.at(K, K - 1) : g.vsh;
   if constexpr (kIdle) pb = act ? pb : g.vsh + 2;
+  // the row update's operands, loaded ahead of the rotation they wait for
+  // (lanes outside the update read an entry of the tile and use none)
+  cx<R> rh1[P], rh2[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    if ((p + 1) * LG - 1 >= K) {
+      const int col = g.idx(p) < NB ? g.idx(p) : NB - 1;
+      rh1[p] = ld<R>(g.at(K, col));
+      rh2[p] = ld<R>(g.at(K + 1, col));
+    }
+  }
   const cx<R> b0 = ld<R>(pb);
   const R b1 = pb[1].x;
   R nb0, nrm2;
@@ -625,12 +603,10 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
     if ((p + 1) * LG - 1 >= K) {
       const int col = g.idx(p);
       if (upd && col >= K && col < n && (TZ || col <= I)) {
-        L2* p1 = g.at(K, col);
-        L2* p2 = g.at(K + 1, col);
-        cx<R> h1 = ld<R>(p1), h2 = ld<R>(p2);
+        cx<R> h1 = rh1[p], h2 = rh2[p];
         rot_left(q, h1, h2);
-        st<R>(p1, h1);
-        st<R>(p2, h2);
+        st<R>(g.at(K, col), h1);
+        st<R>(g.at(K + 1, col), h2);
       }
     }
   }
