@@ -237,20 +237,30 @@ __device__ __forceinline__ int band_qr_one(const Group<double, NB, 1, LAY_BAND, 
   const int r = g.r, n = g.n;
   for (int i = r; i < Gb::kSize; i += Gb::LG) st<double>(g.H + i, mk<double>(0, 0));
   wave_sync();
-  // the Hessenberg part of the compact form, row r
-  if (r < n)
+  // the Hessenberg part of the compact form, row r: at NB <= 16 the loads
+  // are unrolled so that they are all in flight at once
+  if constexpr (NB <= 16) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (r < n && c + 1 >= r && c < n) {
+        const int64_t idx = base + G.el(r, c);
+        st<double>(g.at(r, c), mk<double>(rV[idx], iV[idx]));
+      }
+  } else if (r < n) {
     for (int c = r > 0 ? r - 1 : 0; c < n; ++c) {
       const int64_t idx = base + G.el(r, c);
       st<double>(g.at(r, c), mk<double>(rV[idx], iV[idx]));
     }
+  }
   cx<double> z[1][NB];
 #pragma unroll
   for (int c = 0; c < NB; ++c) z[0][c] = mk<double>(c == r ? 1.0 : 0.0, 0);
   // Q = H_0 H_1 ... H_{n-2}: lane r stages v(r) of reflector i in the
-  // group's vector, then every lane applies Z <- Z - tau (Z v) v^H to its row
-  for (int i = 0; i + 1 < nmax; ++i) {
-    cx<double> vl = mk<double>(r == i + 1 ? 1.0 : 0.0, 0);
-    cx<double> tau = mk<double>(0, 0);
+  // group's vector, then every lane applies Z <- Z - tau (Z v) v^H to its
+  // row.  The next reflector's v(r) and tau are loaded during this one.
+  auto fetch = [&](int i, cx<double> &vl, cx<double> &tau) {
+    vl = mk<double>(r == i + 1 ? 1.0 : 0.0, 0);
+    tau = mk<double>(0, 0);
     if (i + 1 < n) {
       if (r >= i + 2 && r < n) {
         const int64_t idx = base + G.el(r, i);
@@ -258,6 +268,12 @@ __device__ __forceinline__ int band_qr_one(const Group<double, NB, 1, LAY_BAND, 
       }
       tau = mk<double>(rE[vbase + G.ve(i)], iE[vbase + G.ve(i)]);
     }
+  };
+  cx<double> vnext, tnext;
+  fetch(0, vnext, tnext);
+  for (int i = 0; i + 1 < nmax; ++i) {
+    const cx<double> vl = vnext, tau = tnext;
+    if (i + 2 < nmax) fetch(i + 1, vnext, tnext);
     if (r < NB) st<double>(g.vsh + r, vl);
     wave_sync();
     cx<double> w = mk<double>(0, 0);

@@ -184,11 +184,37 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Sum over the NB lanes of a lane group (butterfly, all lanes get the total).
+// x from the lane that DPP control CTRL names within this lane's row of 16
+// (every lane of the row active)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xf, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
+}
+
+// Sum over the NB lanes of a lane group (butterfly, all lanes get the total;
+// every lane of the wavefront active).  Groups of 8 or 16 lanes sit in one
+// DPP row: partners lane^1 and lane^2 (quad permutations), then the mirrored
+// lane within the half row and within the row, in registers -- the same sum
+// in every lane, as each step adds two values that are swapped between the
+// partners.  Wider groups exchange through the LDS crossbar.
 template <int NB, typename R>
 __device__ __forceinline__ R group_sum(R x) {
+  if constexpr (NB == 8 || NB == 16) {
+    x += dpp_mov<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += dpp_mov<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += dpp_mov<0x141>(x);  // row_half_mirror
+    if constexpr (NB == 16) x += dpp_mov<0x140>(x);  // row_mirror
+  } else {
 #pragma unroll
-  for (int o = NB / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, NB);
+    for (int o = NB / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, NB);
+  }
   return x;
 }
 
@@ -408,9 +434,10 @@ __device__ __forceinline__ void gehrd_compact(const Group<double, NB, 1, LAY_DEN
     cx<R> tau = mk<R>(0, 0), scal = mk<R>(0, 0);
     R beta = alpha.re;
     if (live && !(xnorm2 == R(0) && alpha.im == R(0))) {
-      const R nrm = fsqrt(abs2(alpha) + xnorm2);
+      // |beta| and 1/beta from one reciprocal square root
+      const R nrm2 = abs2(alpha) + xnorm2, ir = frsq(nrm2), nrm = nrm2 * ir;
       beta = alpha.re >= R(0) ? -nrm : nrm;
-      const R ib = frcp(beta);
+      const R ib = alpha.re >= R(0) ? -ir : ir;
       tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
       scal = inv(mk(alpha.re - beta, alpha.im));
     }
