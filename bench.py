@@ -116,7 +116,7 @@ def roofline(flops_per_launch, bytes_per_launch, kernel_ms, kernel, traffic_key,
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "roots", "sweep"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "roots", "sweep", "layout"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=0, help="override the configuration's batch size")
@@ -151,6 +151,8 @@ def main():
         out = bench_roots(a, cfg, veig, W, torch, dev, stream)
     elif a.config == "sweep":
         out = bench_sweep(a, cfg, veig, W, torch, dev, stream)
+    elif a.config == "layout":
+        out = bench_layout(a, cfg, veig, torch, dev, stream)
     else:
         out = bench_uniform(a, cfg, veig, W, torch, dev, stream)
     print(json.dumps(out), flush=True)
@@ -470,8 +472,8 @@ def bench_roots(a, cfg, veig, W, torch, dev, stream):
 
 def bench_sweep(a, cfg, veig, W, torch, dev, stream):
     """The reference driver's sweep (zlahqr.cpp:196-252: 13,312 matrices per
-    order, one batch per order) over the orders this library takes,
-    10..64: FP64 matrices/s per order; `value` = all matrices / all time."""
+    order, one batch per order, orders 10..79): FP64 matrices/s per order;
+    `value` = all matrices / all time."""
     N, T = cfg["batch"], 64
     rates, total_ms = {}, 0.0
     for n in range(cfg["nmin"], cfg["nmax"] + 1):
@@ -493,7 +495,7 @@ def bench_sweep(a, cfg, veig, W, torch, dev, stream):
         # four of the orders (whole batch, timed once after a warm-up call)
         lib, kind, cores = _cpu_lib()
         per = {}
-        for n in (16, 32, 48, 64):
+        for n in (16, 32, 48, 64, 79):
             if not cfg["nmin"] <= n <= cfg["nmax"]:
                 continue
             rA, iA = W.gaussian_soa(N, n, T, cfg["seed"] + n, dev)
@@ -511,6 +513,43 @@ def bench_sweep(a, cfg, veig, W, torch, dev, stream):
             "config": {"workload": f"{N} complex128 Gaussian matrices per order, tile {T}, one launch per order "
                                    f"(the reference driver's sweep sizes)", "name": "sweep", "batch": N},
             "roofline": None, "cpu_baseline": cpu, "per_order": rates}
+
+
+def bench_layout(a, cfg, veig, torch, dev, stream):
+    """The device layout converters (the reference's rearrange_matrices,
+    arrange_matrices_aos, rep_matrix_soa, rep_vector_soa; utils.cpp) on one
+    float64 plane of the c2 batch: HBM bytes moved per second (a read and a
+    write of every element; the replicators read one matrix or vector)."""
+    n, N, T = cfg["n"], cfg["batch"], 64
+    bc = N // T
+    na, ne = N * n * n, N * n
+    f64 = dict(dtype=torch.float64, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(cfg["seed"])
+    src = torch.randn(na, generator=g, **f64)
+    dst = torch.empty(na, **f64)
+    one = torch.randn(n * n, generator=g, **f64)
+    res = {}
+    for kind, s_, d_, nbytes in (("soa", src, dst, 16 * na), ("aos", src, dst, 16 * na),
+                                 ("repm", one, dst, 8 * na), ("repv", one[:n], dst[:ne], 8 * ne)):
+        ms, _ = timed(lambda: veig.layout(kind, n, bc, T, s_, d_, stream=stream), a.steps, a.warmup, stream)
+        res[kind] = {"ms": round(ms, 4), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1), "bytes": nbytes}
+        progress(f"layout {kind}: {res[kind]['GB/s']} GB/s")
+    # parity: the AoS -> SoA -> AoS round trip is the identity
+    veig.layout("soa", n, bc, T, src, dst, stream=stream)
+    back = torch.empty_like(src)
+    veig.layout("aos", n, bc, T, dst, back, stream=stream)
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(back, src))
+    v = res["soa"]["GB/s"]
+    return {"metric": "layout conversion GB/s (AoS -> tiled SoA, one float64 plane)", "value": v, "unit": "GB/s",
+            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": res["soa"]["ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{N} order-{n} matrices, one float64 plane ({8 * na} bytes), tile {T}",
+                       "name": "layout", "batch": N, "n": n},
+            "roofline": {"bound": "hbm", "achieved": v, "peak": 8000.0, "unit": "GB/s", "frac": round(v / 8000.0, 4),
+                         "traffic": None, "kernel": "k_transpose (64 x 64 LDS tiles)"},
+            "cpu_baseline": None, "kinds": res, "round_trip_exact": exact}
 
 
 def cpu_baseline_roots(rC, iC, n, T, N, target_s):

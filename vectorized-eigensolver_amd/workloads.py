@@ -31,7 +31,9 @@ CONFIGS = {
     # complex Gaussian coefficients c_0..c_16
     "roots": dict(n=16, batch=1 << 20, seed=1006),
     # the reference driver's sweep (zlahqr.cpp:196-252): 13,312 matrices per order
-    "sweep": dict(nmin=10, nmax=64, batch=13312, seed=1100),
+    "sweep": dict(nmin=10, nmax=79, batch=13312, seed=1100),
+    # the layout converters on one float64 plane of the c2 batch
+    "layout": dict(n=16, batch=1 << 20, seed=1107),
 }
 
 
@@ -61,14 +63,24 @@ def coeff_soa(N: int, n: int, tile: int, seed: int, device="cuda"):
 
 
 def haar_unitary(N: int, n: int, g, device):
-    """Haar-distributed unitary matrices: the Q of a complex Gaussian matrix
-    with the phases of R's diagonal moved into Q (Mezzadri, "How to generate
-    random matrices from the classical compact groups", 2007)."""
+    """Haar-distributed unitary matrices: the Q of the QR factorisation of a
+    complex Gaussian matrix whose R has a positive real diagonal (Mezzadri,
+    "How to generate random matrices from the classical compact groups",
+    2007).  Q comes from Gram-Schmidt with reorthogonalisation (each column
+    projected out twice), batched over the N matrices -- its R diagonal is
+    the positive column norm, so no phase correction is needed -- rather than
+    a batched library QR, which is slow for many small matrices on the GPU."""
     Z = torch.complex(torch.randn(N, n, n, dtype=torch.float64, device=device, generator=g),
                       torch.randn(N, n, n, dtype=torch.float64, device=device, generator=g))
-    Q, R = torch.linalg.qr(Z)
-    d = torch.diagonal(R, dim1=-2, dim2=-1)
-    return Q * (d / d.abs()).unsqueeze(-2)
+    Q = torch.empty_like(Z)
+    for j in range(n):
+        v = Z[:, :, j:j + 1]
+        if j:
+            P = Q[:, :, :j]
+            for _ in range(2):
+                v = v - P @ (P.conj().transpose(-2, -1) @ v)
+        Q[:, :, j:j + 1] = v / torch.linalg.vector_norm(v, dim=1, keepdim=True)
+    return Q
 
 
 def conditioned(N: int, n: int, cond, g, device, lam=None):
