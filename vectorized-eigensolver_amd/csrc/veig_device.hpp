@@ -87,6 +87,20 @@ __device__ __forceinline__ double frsq(double x) {
 }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// One Newton step from the seed: relative error ~1.5 (2^-24.2)^2 = 2^-47.8,
+// for quantities that steer the iteration without entering the similarity
+// transform (the shift), where the second step buys nothing measurable
+__device__ __forceinline__ double frcp1(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+__device__ __forceinline__ float frcp1(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frsq1(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return fma(y, fma(-0.5 * x * y, y, 0.5), y);
+}
+__device__ __forceinline__ float frsq1(float x) { return __builtin_amdgcn_rsqf(x); }
+
 __device__ __forceinline__ double fsqrt(double x) {
   // s = x*rsq(x), then one Newton correction on s itself
   if (!(x > 0.0)) return x == 0.0 ? 0.0 : __builtin_nan("");
@@ -152,12 +166,12 @@ template <typename R> __device__ __forceinline__ cx<R> cldexp(cx<R> a, int e) {
 // principal square root of an a with |a| of order one (the scaled Wilkinson
 // discriminant): t = sqrt((|a| + |Re a|) / 2) = q rsq(q) and |Im a| / (2 t) =
 // |Im a| rsq(q) / 2, two reciprocal square roots in place of two square roots
-// and a reciprocal; 0 for a = 0
+// and a reciprocal (one Newton step each: it feeds the shift only); 0 for a = 0
 template <typename R> __device__ __forceinline__ cx<R> csqrt_unit(cx<R> a) {
   const R m2 = abs2(a);
-  const R m = m2 > R(0) ? m2 * frsq(m2) : R(0);
+  const R m = m2 > R(0) ? m2 * frsq1(m2) : R(0);
   const R q = R(0.5) * (m + fabs(a.re));
-  const R rq = q > R(0) ? frsq(q) : R(0);
+  const R rq = q > R(0) ? frsq1(q) : R(0);
   const R zr = q * rq, zi = fabs(a.im) * (R(0.5) * rq);
   cx<R> o = (a.re < R(0)) ? mk(zi, zr) : mk(zr, zi);
   if (a.im < R(0)) o.im = -o.im;
@@ -853,7 +867,9 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
           const cx<R> xs = cldexp(x, -E);
           cx<R> ys = csqrt_unit(add(mul(xs, xs), cldexp(u2, -2 * E)));  // y / 2^E
           if (sx > R(0) && xs.re * ys.re + xs.im * ys.im < R(0)) ys = mk(-ys.re, -ys.im);
-          t = sub(t, mul(cldexp(u2, -E), inv(add(xs, ys))));
+          const cx<R> d = add(xs, ys);
+          const R id = frcp1(abs2(d));
+          t = sub(t, mul(cldexp(u2, -E), mk(d.re * id, -d.im * id)));
         }
       }
       const cx<R> h11s = sub(ld<R>(g.at(L, L)), t);
@@ -873,12 +889,13 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
 
     // ---- wavefront-uniform range of bulge positions ----
     int kmin = NB, kmax = 0;
+    // one readlane per group: (L, I) packed, -1 when the group is not ready
+    const int packed = ready ? ((L << 8) | I) : -1;
 #pragma unroll
     for (int q = 0; q < MPW; ++q) {
-      const int rq = __builtin_amdgcn_readlane(ready ? 1 : 0, q * LG);
-      const int lq = __builtin_amdgcn_readlane(L, q * LG);
-      const int iq = __builtin_amdgcn_readlane(I, q * LG);
-      if (rq) {
+      const int pq = __builtin_amdgcn_readlane(packed, q * LG);
+      if (pq >= 0) {
+        const int lq = pq >> 8, iq = pq & 255;
         kmin = lq < kmin ? lq : kmin;
         kmax = iq > kmax ? iq : kmax;
       }
