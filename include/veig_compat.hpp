@@ -97,3 +97,27 @@ void test_zunghr(double *rA, double *iA, double *rtau, double *itau, double *rwo
                  int dim, int block_cnt, int thread_cnt);
 void test_backsubstitution(double *rT, double *iT, double *rx, double *ix, int dim, int block_cnt,
                            int thread_cnt);
+
+// Per-matrix entries of the header library (Toolbox.hpp:1315, 1492, 690,
+// 1133): ONE matrix of the tiled layout, the pointers pre-offset to its slot
+// and its elements thread_cnt apart (element (r, c) of the dim x dim matrix
+// at p[(r + c*dim) * thread_cnt], vector entry i at p[i * thread_cnt]), as
+// the batch entries above pass them.  Each call gathers the matrix, runs the
+// batch entry on it as a batch of one on the GPU and scatters the results
+// back, so its values are the batch entry's for that slot; a caller that
+// loops over slots should call the batch entry instead (one launch instead
+// of one per matrix).  The work arrays are accepted and unused.
+void zgehrd(double *rA, double *iA, double *rtau, double *itau, double *rwork, double *iwork, int dim,
+            int block_cnt, int thread_cnt);
+void zunghr(double *rA, double *iA, double *rtau, double *itau, double *rwork, double *iwork, int dim,
+            int block_cnt, int thread_cnt);
+// wantT = false computes T in full as well (the reference leaves the entries
+// outside the active block unupdated; every reference caller passes true);
+// wantZ = false leaves Z untouched.
+void zlahqr_schur(bool wantT, bool wantZ, int dim, double *rV, double *iV, double *rE, double *iE,
+                  double *rZ, double *iZ, int block_cnt, int thread_cnt);
+// right eigenvectors into R (which holds the Schur vectors on entry); T is
+// left as it is; L is not touched (the reference's left eigenvectors are
+// unimplemented, its batch entry passes none, Toolbox.hpp:2079)
+void ztrevc(double *rT, double *iT, double *rL, double *iL, double *rR, double *iR, double *rwork,
+            double *iwork, int dim, int block_cnt, int thread_cnt);

@@ -10,6 +10,7 @@
 //   dabs2, dabs3  scaled 2- and 3-norms                               (393-430)
 //   cabs1 = |re| + |im|; cabs = sqrt(re^2 + im^2) unscaled            (432-449)
 //   csqrt    principal square root                                  (451-488)
+// and the header library's row/column scaling scale_rc (Toolbox.hpp:14-29).
 #pragma once
 
 #include <cmath>
@@ -124,6 +125,22 @@ VEIG_HD void csqrt(double &r, double &i) {
 
 // The value type of the reference (ComplexDouble.hpp:110-281): a pair of
 // doubles with the usual compound operators built on the functions above.
+// scale_rc (Toolbox.hpp:14-29), the step that makes a subdiagonal entry
+// real: in the dim x dim column-major matrix whose elements lie n apart
+// from (rV, iV), row I times z over columns I+1..I2, and column I times
+// conj(z) over rows I1..I-1 (1-based indices, as in the reference)
+VEIG_HD void scale_rc(double *rV, double *iV, double zr, double zi, int I, int I1, int I2, int dim, int n) {
+  const long long ld = (long long)dim * n;
+  for (int c = I + 1; c <= I2; ++c) {
+    const long long k = (long long)(I - 1) * n + (long long)(c - 1) * ld;
+    cmul(rV[k], iV[k], zr, zi);
+  }
+  for (int r = I1; r <= I - 1; ++r) {
+    const long long k = (long long)(r - 1) * n + (long long)(I - 1) * ld;
+    cmulc(rV[k], iV[k], zr, zi);
+  }
+}
+
 struct ComplexDouble {
   double r, i;
   VEIG_HD ComplexDouble(double real, double imag) : r(real), i(imag) {}

@@ -20,3 +20,16 @@ extern "C" int probe_device(int op, int64_t n, const double *a, double *out) {
   if (e == hipSuccess) e = hipDeviceSynchronize();
   return (int)e;
 }
+
+__global__ void k_scale_rc(double *rV, double *iV, double zr, double zi, int I, int I1, int I2, int dim, int n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) scale_rc(rV, iV, zr, zi, I, I1, I2, dim, n);
+}
+
+// scale_rc in device code on device planes; returns a hipError_t
+extern "C" int probe_scale_rc_device(double *rV, double *iV, double zr, double zi, int I, int I1, int I2, int dim,
+                                     int n) {
+  hipLaunchKernelGGL(k_scale_rc, dim3(1), dim3(64), 0, 0, rV, iV, zr, zi, I, I1, I2, dim, n);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  return (int)e;
+}

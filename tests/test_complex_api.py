@@ -162,3 +162,48 @@ def test_products_and_value_type(run):
     assert np.allclose(v[:, 0] + 1j * v[:, 1], ((x + y) * y - x) / y / 2, rtol=1e-13, atol=1e-13)
     w = run("cd_inv_sqrt", a)
     assert np.allclose(w[:, 0] + 1j * w[:, 1], np.sqrt(1 / x), rtol=1e-14)
+
+
+def _scale_rc_host(rV, iV, zr, zi, I, I1, I2, dim, n):
+    L = _host()
+    L.probe_scale_rc_host.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double] + \
+        [ctypes.c_int] * 5
+    L.probe_scale_rc_host(rV.ctypes.data, iV.ctypes.data, zr, zi, I, I1, I2, dim, n)
+    return rV, iV
+
+
+def _scale_rc_device(rV, iV, zr, zi, I, I1, I2, dim, n):
+    import torch
+    L = ctypes.CDLL(str(DEV_LIB))
+    L.probe_scale_rc_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double] + \
+        [ctypes.c_int] * 5
+    tr, ti = torch.from_numpy(rV).cuda(), torch.from_numpy(iV).cuda()
+    torch.cuda.synchronize()
+    assert L.probe_scale_rc_device(tr.data_ptr(), ti.data_ptr(), zr, zi, I, I1, I2, dim, n) == 0
+    return tr.cpu().numpy(), ti.cpu().numpy()
+
+
+@pytest.mark.parametrize("run", [pytest.param(_scale_rc_host, id="host"),
+                                 pytest.param(_scale_rc_device, id="gfx950", marks=pytest.mark.gpu)])
+@pytest.mark.parametrize("dim,n,I,I1,I2", [(6, 1, 3, 1, 6), (6, 4, 4, 2, 5), (9, 3, 1, 1, 9), (9, 3, 9, 1, 9),
+                                           (5, 2, 2, 2, 2)])
+def test_scale_rc(run, dim, n, I, I1, I2):
+    """scale_rc (Toolbox.hpp:14-29) in the strided layout: row I (1-based)
+    over columns I+1..I2 times z, column I over rows I1..I-1 times conj(z),
+    every other element untouched -- including the other slots between the
+    matrix's elements."""
+    rng = np.random.default_rng(dim * 100 + I)
+    M = rng.standard_normal((dim * dim * n,)) + 1j * rng.standard_normal((dim * dim * n,))
+    z = complex(0.6, -0.8)
+    rV, iV = run(M.real.copy(), M.imag.copy(), z.real, z.imag, I, I1, I2, dim, n)
+    want = M.copy()
+    idx = lambda r, c: ((r - 1) + (c - 1) * dim) * n  # 1-based (r, c) of the slot at offset 0
+    for c in range(I + 1, I2 + 1):
+        want[idx(I, c)] *= z
+    for r in range(I1, I):
+        want[idx(r, I)] *= np.conj(z)
+    got = rV + 1j * iV
+    assert np.allclose(got, want, rtol=5e-16, atol=0)  # one rounding of a complex product
+    changed = np.flatnonzero(got != M)
+    touched = {idx(I, c) for c in range(I + 1, I2 + 1)} | {idx(r, I) for r in range(I1, I)}
+    assert set(changed.tolist()) <= touched

@@ -255,3 +255,76 @@ void test_backsubstitution(double *rT, double *iT, double *rx, double *ix, int d
                            int thread_cnt) {
   call_backsubstitution(dim, block_cnt, thread_cnt, rT, iT, rx, ix);
 }
+
+// ---------------------------------------------------------------------------
+// per-matrix entries (Toolbox.hpp:1315, 1492, 690, 1133)
+// ---------------------------------------------------------------------------
+namespace {
+// the count elements of one slot, stride apart from (re, im), gathered into
+// contiguous planes (the tiled layout with thread_cnt = 1) and scattered back
+struct Slot {
+  std::vector<double> re, im;
+  double *pr, *pi;
+  int64_t stride;
+  Slot(double *r, double *i, size_t count, int s, const char *what) : re(count), im(count), pr(r), pi(i), stride(s) {
+    if (count && (!r || !i)) throw veig::Error(VEIG_ERR_INVALID, std::string(what) + ": null pointer");
+    for (size_t k = 0; k < count; ++k) {
+      re[k] = pr[(int64_t)k * stride];
+      im[k] = pi[(int64_t)k * stride];
+    }
+  }
+  void back() {
+    for (size_t k = 0; k < re.size(); ++k) {
+      pr[(int64_t)k * stride] = re[k];
+      pi[(int64_t)k * stride] = im[k];
+    }
+  }
+};
+
+bool slot_args(int dim, int thread_cnt, const char *what) {
+  if (thread_cnt <= 0) throw veig::Error(VEIG_ERR_INVALID, std::string(what) + ": thread_cnt must be positive");
+  return dim > 0;  // an empty matrix: nothing to do, as in the reference
+}
+}  // namespace
+
+void zgehrd(double *rA, double *iA, double *rtau, double *itau, double *, double *, int dim, int,
+            int thread_cnt) {
+  if (!slot_args(dim, thread_cnt, "zgehrd")) return;
+  Slot a(rA, iA, (size_t)dim * dim, thread_cnt, "zgehrd"), t(rtau, itau, (size_t)dim, thread_cnt, "zgehrd");
+  call_zgehrd(dim, 1, 1, a.re.data(), a.im.data(), t.re.data(), t.im.data());
+  a.back();
+  t.back();
+}
+
+void zunghr(double *rA, double *iA, double *rtau, double *itau, double *, double *, int dim, int,
+            int thread_cnt) {
+  if (!slot_args(dim, thread_cnt, "zunghr")) return;
+  Slot a(rA, iA, (size_t)dim * dim, thread_cnt, "zunghr"), t(rtau, itau, (size_t)dim, thread_cnt, "zunghr");
+  call_zunghr(dim, 1, 1, a.re.data(), a.im.data(), t.re.data(), t.im.data());
+  a.back();
+}
+
+void zlahqr_schur(bool, bool wantZ, int dim, double *rV, double *iV, double *rE, double *iE, double *rZ,
+                  double *iZ, int, int thread_cnt) {
+  if (!slot_args(dim, thread_cnt, "zlahqr_schur")) return;
+  const size_t nn = (size_t)dim * dim;
+  Slot v(rV, iV, nn, thread_cnt, "zlahqr_schur"), e(rE, iE, (size_t)dim, thread_cnt, "zlahqr_schur");
+  std::vector<double> zr(nn, 0.0), zi(nn, 0.0);
+  Slot *z = nullptr;
+  Slot zs = wantZ ? Slot(rZ, iZ, nn, thread_cnt, "zlahqr_schur") : Slot(nullptr, nullptr, 0, 1, "zlahqr_schur");
+  if (wantZ) z = &zs;
+  double *pzr = z ? z->re.data() : zr.data(), *pzi = z ? z->im.data() : zi.data();
+  zlahqr_new(e.re.data(), e.im.data(), v.re.data(), v.im.data(), pzr, pzi, dim, dim * dim, 1, 1);
+  v.back();
+  e.back();
+  if (z) z->back();
+}
+
+void ztrevc(double *rT, double *iT, double *, double *, double *rR, double *iR, double *, double *, int dim, int,
+            int thread_cnt) {
+  if (!slot_args(dim, thread_cnt, "ztrevc")) return;
+  const size_t nn = (size_t)dim * dim;
+  Slot t(rT, iT, nn, thread_cnt, "ztrevc"), r(rR, iR, nn, thread_cnt, "ztrevc");
+  call_ztrevc(dim, 1, 1, t.re.data(), t.im.data(), r.re.data(), r.im.data());
+  r.back();
+}

@@ -101,6 +101,15 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise VeigError(f"{LIB_PATH} is not built; run vectorized-eigensolver_amd/build.py")
+        # torch ships its own HIP runtime under the same soname as the system
+        # one that libveig.so is linked against: whichever loads first serves
+        # the whole process, and torch does not find the GPU on a runtime
+        # other than its own.  Load torch's first (where torch is installed),
+        # so that the library and torch share it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(str(LIB_PATH))
         for name, (argt, rest) in _SIGS.items():
             if hasattr(L, name):
