@@ -18,8 +18,11 @@ import veig
 pytestmark = pytest.mark.gpu
 
 DP = ctypes.POINTER(ctypes.c_double)
-DIM, BC, TC = 7, 2, 4
+BC, TC = 2, 4
 SLOTS = [(0, 0), (1, 2), (1, 3)]          # (block, thread)
+# orders on the lane-group stage kernels (1, 7, 16) and on the one-workgroup
+# kernels (40)
+DIMS = [1, 7, 16, 40]
 
 
 def _fn(name, argtypes):
@@ -34,11 +37,11 @@ def _at(a: np.ndarray, off: int):
     return ctypes.cast(a.ctypes.data + 8 * off, DP)
 
 
-def _moff(b, t):
+def _moff(b, t, DIM):
     return t + b * TC * DIM * DIM
 
 
-def _voff(b, t):
+def _voff(b, t, DIM):
     return t + b * TC * DIM
 
 
@@ -53,17 +56,18 @@ def _planes(A):
     return [np.ascontiguousarray(x) for x in H.to_soa(A, TC)]
 
 
-def _vec(N, seed):
+def _vec(N, DIM, seed):
     rng = np.random.default_rng(seed)
     return rng.standard_normal(N * DIM), rng.standard_normal(N * DIM)
 
 
+@pytest.mark.parametrize("DIM", DIMS)
 @pytest.mark.parametrize("b,t", SLOTS)
-def test_zgehrd_zunghr_slot(b, t):
+def test_zgehrd_zunghr_slot(b, t, DIM):
     N = BC * TC
     A = H.gaussian(N, DIM, 2101)
     rA0, iA0 = _planes(A)
-    rt0, it0 = _vec(N, 2102)                 # tau planes: entries past the reflectors stay
+    rt0, it0 = _vec(N, DIM, 2102)                 # tau planes: entries past the reflectors stay
     batch_g = _fn("_Z11test_zgehrdPdS_S_S_S_S_iii", [DP] * 6 + [ctypes.c_int] * 3)
     batch_q = _fn("_Z11test_zunghrPdS_S_S_S_S_iii", [DP] * 6 + [ctypes.c_int] * 3)
     slot_g = _fn("_Z6zgehrdPdS_S_S_S_S_iii", [DP] * 6 + [ctypes.c_int] * 3)
@@ -75,7 +79,7 @@ def test_zgehrd_zunghr_slot(b, t):
     batch_q(*(H.dptr(x) for x in (rA, iA, rt, it)), None, None, DIM, BC, TC)
     # one slot
     sA, siA, st, sit = rA0.copy(), iA0.copy(), rt0.copy(), it0.copy()
-    mo, vo = _moff(b, t), _voff(b, t)
+    mo, vo = _moff(b, t, DIM), _voff(b, t, DIM)
     slot_g(_at(sA, mo), _at(siA, mo), _at(st, vo), _at(sit, vo), None, None, DIM, BC, TC)
     mm, mv = _slot_mask(DIM * DIM, b, t), _slot_mask(DIM, b, t)
     assert np.array_equal(sA[mm], rH[mm]) and np.array_equal(siA[mm], iH[mm])
@@ -88,23 +92,24 @@ def test_zgehrd_zunghr_slot(b, t):
     assert np.allclose(Q.conj().T @ Q, np.eye(DIM), atol=1e-14)
 
 
+@pytest.mark.parametrize("DIM", DIMS)
 @pytest.mark.parametrize("b,t", SLOTS)
 @pytest.mark.parametrize("want_z", [True, False])
-def test_zlahqr_schur_slot(b, t, want_z):
+def test_zlahqr_schur_slot(b, t, want_z, DIM):
     N = BC * TC
     Hm = H.hessenberg_input(N, DIM, 2103)
     rng = np.random.default_rng(2104)
     Z0 = rng.standard_normal((N, DIM, DIM)) + 1j * rng.standard_normal((N, DIM, DIM))
     rV0, iV0 = _planes(Hm)
     rZ0, iZ0 = _planes(Z0)
-    rE0, iE0 = _vec(N, 2105)
+    rE0, iE0 = _vec(N, DIM, 2105)
     batch = _fn("_Z10zlahqr_newPdS_S_S_S_S_iiii", [DP] * 6 + [ctypes.c_int] * 4)
     slot = _fn("_Z12zlahqr_schurbbiPdS_S_S_S_S_ii", [ctypes.c_bool, ctypes.c_bool, ctypes.c_int] + [DP] * 6 +
                [ctypes.c_int] * 2)
     rV, iV, rZ, iZ, rE, iE = (x.copy() for x in (rV0, iV0, rZ0, iZ0, rE0, iE0))
     batch(*(H.dptr(x) for x in (rE, iE, rV, iV, rZ, iZ)), DIM, DIM * DIM, BC, TC)
     sV, siV, sZ, siZ, sE, siE = (x.copy() for x in (rV0, iV0, rZ0, iZ0, rE0, iE0))
-    mo, vo = _moff(b, t), _voff(b, t)
+    mo, vo = _moff(b, t, DIM), _voff(b, t, DIM)
     slot(True, want_z, DIM, _at(sV, mo), _at(siV, mo), _at(sE, vo), _at(siE, vo), _at(sZ, mo), _at(siZ, mo), BC, TC)
     mm, mv = _slot_mask(DIM * DIM, b, t), _slot_mask(DIM, b, t)
     assert np.array_equal(sV[mm], rV[mm]) and np.array_equal(siV[mm], iV[mm])
@@ -116,8 +121,9 @@ def test_zlahqr_schur_slot(b, t, want_z):
     assert np.array_equal(sV[~mm], rV0[~mm]) and np.array_equal(sE[~mv], rE0[~mv])
 
 
+@pytest.mark.parametrize("DIM", DIMS)
 @pytest.mark.parametrize("b,t", SLOTS)
-def test_ztrevc_slot(b, t):
+def test_ztrevc_slot(b, t, DIM):
     N = BC * TC
     A = H.gaussian(N, DIM, 2106)
     E, V, T = H.oracle_eig(A, TC)
@@ -131,7 +137,7 @@ def test_ztrevc_slot(b, t):
     rT, iT, rR, iR = (x.copy() for x in (rT0, iT0, rR0, iR0))
     batch(*(H.dptr(x) for x in (rT, iT, rR, iR)), None, None, DIM, DIM * DIM, BC, TC)
     sT, siT, sR, siR = (x.copy() for x in (rT0, iT0, rR0, iR0))
-    mo = _moff(b, t)
+    mo = _moff(b, t, DIM)
     slot(_at(sT, mo), _at(siT, mo), None, None, _at(sR, mo), _at(siR, mo), None, None, DIM, BC, TC)
     mm = _slot_mask(DIM * DIM, b, t)
     assert np.array_equal(sR[mm], rR[mm]) and np.array_equal(siR[mm], iR[mm])
