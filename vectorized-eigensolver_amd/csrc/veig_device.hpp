@@ -587,6 +587,10 @@ __device__ __forceinline__ void rot_right(const Rot<R>& q, cx<R>& h1, cx<R>& h2)
 // budget fails to compile, it cannot miscompile).
 template <int K> struct ZPin;
 template <int C> struct ZPinMulc;
+// where the fixed registers apply: FP64, order 16 on the banded tile, several
+// matrices per wavefront (the kernels held to three wavefronts per SIMD)
+template <typename R, int NB, int P, int LAY, int MPW>
+inline constexpr bool kPinnedRow = std::is_same_v<R, double> && NB == 16 && P == 1 && LAY == LAY_BAND && MPW > 1;
 // ZPin<K>: [z[K], z[K+1]] <- [z[K], z[K+1]] G, G = [c, -s; s, conj(c)]:
 // each output is c.re times its own entry plus a two-product partial sum
 #define VEIG_ZPIN(K, A, B, C_, D, E, F, G_, H)                                                      \
@@ -745,7 +749,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
   if (TZ && upd) {
     auto zupd = [&](auto kc) {
       constexpr int KK = decltype(kc)::value;
-      if constexpr (NB == 16 && P == 1 && LAY == LAY_BAND && MPW > 1) {
+      if constexpr (kPinnedRow<R, NB, P, LAY, MPW>) {
         ZPin<KK>::rot(q.c.re, q.c.im, q.s, z[0][KK].re, z[0][KK].im, z[0][KK + 1].re, z[0][KK + 1].im);
       } else {
 #pragma unroll
@@ -1018,7 +1022,7 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
 #pragma unroll
           for (int c = 1; c < NB; ++c)
             if (c == I) {
-              if constexpr (NB == 16 && P == 1 && LAY == LAY_BAND) {
+              if constexpr (kPinnedRow<R, NB, P, LAY, MPW>) {
                 static_for<NB>([&](auto cc) {
                   constexpr int C = decltype(cc)::value;
                   if (C == c) ZPinMulc<C>::mul(zz.re, zz.im, z[0][C].re, z[0][C].im);
