@@ -178,13 +178,15 @@ __device__ __forceinline__ void band_reduce_one(ReduceTiles<NB, LW> &sm, int n, 
   const int r = g.r, c0 = wq * CH;
   cx<double> tau = mk<double>(0, 0);
   if constexpr (reduce_waves<NB>() == 1) {
-    // one wavefront: runtime loops over the row (unrolled loads of the whole
-    // row would be hoisted together and cost registers)
+    // one wavefront: the whole row in flight at once (this kernel's LDS,
+    // not its registers, limits its occupancy; a runtime loop over the row
+    // waited for each pair of loads in turn: 6% of its time), entries
+    // outside the matrix as zeros
+    cx<double> row[1][NB];
+    load_row_regs<double, NB, 1>(g, G, base, rA, iA, row);
 #pragma unroll
-    for (int c = 0; c < NB; ++c) st<double>(g.at(r, c), mk<double>(0, 0));
+    for (int c = 0; c < NB; ++c) st<double>(g.at(r, c), row[0][c]);
     st<double>(g.vsh + r, mk<double>(0, 0));
-    wave_sync();
-    load_rows<double, NB, 1>(g, G, base, rA, iA, NB);
     wave_sync();
     gehrd_compact<NB>(g, nmax, tau);
     store_rows<double, NB, 1>(g, G, base, rV, iV);

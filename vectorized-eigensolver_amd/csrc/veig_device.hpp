@@ -707,8 +707,15 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
     }
   }
   // H(K, K-1) = r, H(K+1, K-1) = 0; at K = L this overwrites the staged
-  // first column, which lahqr clears after the sweep
-  if (act && K > 0 && g.r == 0) {
+  // first column, which lahqr clears after the sweep.  With several matrices
+  // per wavefront every lane writes its group's pb, without a branch: a
+  // group outside its sweep rewrites its identity input unchanged (r = 1
+  // exactly), and at K = 0 pb is the staged column in vsh, read only above
+  if constexpr (kIdle) {
+    L2* pk = const_cast<L2*>(pb);
+    st<R>(pk, mk<R>(nb0, R(0)));
+    st<R>(pk + 1, mk<R>(0, 0));
+  } else if (act && K > 0 && g.r == 0) {
     L2* pk = g.at(K, K > 0 ? K - 1 : 0);
     pk[0].x = nb0;
     pk[0].y = R(0);
@@ -766,12 +773,17 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
   wave_sync();
 }
 
+// The unrolled sweep over K = KEND-1 ... 0 as bit masks: bit K of the
+// wavefront-uniform kmask says some group's sweep passes step K (one scalar
+// bit test per step), bit K of the lane's amask that its own group's does
+// (one vector bit test): the range compares they replace cost 2% of the
+// order-16 step in scalar instructions alone.
 template <int K, int KEND, bool TZ = true, typename R, int NB, int P, int LAY, int LW>
-__device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], bool ready,
-                                            int L, int I, int kmin, int kmax) {
+__device__ __forceinline__ void sweep_range(const Group<R, NB, P, LAY, LW>& g, cx<R> (&z)[P][NB], unsigned amask,
+                                            int L, int I, unsigned kmask) {
   if constexpr (K < KEND) {
-    if (K >= kmin && K < kmax) sweep_step<K, TZ>(g, z, ready && K >= L && K < I, L, I);
-    sweep_range<K + 1, KEND, TZ>(g, z, ready, L, I, kmin, kmax);
+    if ((kmask >> K) & 1u) sweep_step<K, TZ>(g, z, (amask >> K) & 1u, L, I);
+    sweep_range<K + 1, KEND, TZ>(g, z, amask, L, I, kmask);
   }
 }
 
@@ -991,7 +1003,11 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
       static_assert(P == 1, "one matrix per wavefront uses one index per lane");
       for (int K = kmin; K < kmax; ++K) sweep_step<-1, TZ>(g, z, ready && K >= L && K < I, L, I, K);
     } else {
-      sweep_range<0, NB - 1, TZ>(g, z, ready, L, I, kmin, kmax);
+      // steps kmin..kmax-1 of the wavefront, L..I-1 of this group
+      static_assert(NB <= 32, "step masks are 32 bits");
+      const unsigned kmask = kmax > kmin ? ((1u << kmax) - 1u) & ~((1u << kmin) - 1u) : 0u;
+      const unsigned amask = ready ? ((1u << I) - 1u) & ~((1u << L) - 1u) : 0u;
+      sweep_range<0, NB - 1, TZ>(g, z, amask, L, I, kmask);
     }
     if (ready && L > 0 && g.r == 0) {  // the staged first column
       st<R>(g.at(L, L - 1), mk<R>(0, 0));
