@@ -578,11 +578,12 @@ __device__ __forceinline__ void rot_right(const Rot<R>& q, cx<R>& h1, cx<R>& h2)
 // The Schur-vector update of the unrolled banded sweep at order 16 with
 // row entry z[k] in fixed registers (re in v[104+4k : 105+4k], im in
 // v[106+4k : 107+4k]): each step rotates columns K, K+1 in place, and
-// making H(I, I-1) real scales z[I] in place, so the row keeps one set of
-// registers from one sweep to the next instead of migrating through fresh
-// ones and being permuted back at the end of every sweep (30 moves per
-// sweep; measured 0.45% of the c2 step).  The arithmetic is rot_right's and
-// mulc's, operation for operation, so the results are bit-identical.
+// making H(I, I-1) real scales z[I] in place, so within a sweep the row
+// stays in one set of registers instead of migrating through fresh ones
+// (measured 0.45% of the order-16 step; the allocator still keeps the row
+// elsewhere between sweeps, about 60 moves per sweep).  The arithmetic is
+// rot_right's and mulc's, operation for operation, so the results are
+// bit-identical.
 // Requires the 168-register budget of three wavefronts per SIMD (a smaller
 // budget fails to compile, it cannot miscompile).
 template <int K> struct ZPin;
