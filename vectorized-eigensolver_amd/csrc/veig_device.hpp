@@ -875,24 +875,28 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
         const int k = g.idx(p);
         if (need && its <= itmax && k > 0 && k <= I) {
           const cx<R> h = ld<R>(g.at(k, k - 1));
-          if (cabs1(h) <= smlnum) {
-            neg = true;
-          } else {
-            const cx<R> hkk1 = ld<R>(g.at(k - 1, k - 1)), hkk = ld<R>(g.at(k, k));
-            R tst = cabs1(hkk1) + cabs1(hkk);
+          const cx<R> hkk1 = ld<R>(g.at(k - 1, k - 1)), hkk = ld<R>(g.at(k, k));
+          R tst = cabs1(hkk1) + cabs1(hkk);
+          // the rare zero diagonal pair behind a wavefront-uniform test, and
+          // a subdiagonal entry below smlnum (negligible outright) folded
+          // into the result: fewer scalar instructions than the reference's
+          // nested branches, the same outcome
+          if (__builtin_expect(__any(tst == R(0)), 0)) {
             if (tst == R(0)) {
               if (k - 2 >= 0) tst += fabs(ld<R>(g.at(k - 1, k - 2)).re);
               if (k + 1 <= n - 1) tst += fabs(ld<R>(g.at(k + 1, k)).re);
             }
-            if (fabs(h.re) <= ulp * tst) {
-              const cx<R> hup = ld<R>(g.at(k - 1, k));
-              const R ab = fmax(cabs1(h), cabs1(hup)), ba = fmin(cabs1(h), cabs1(hup));
-              const R d3 = cabs1(sub(hkk1, hkk));
-              const R aa = fmax(cabs1(hkk), d3), bb = fmin(cabs1(hkk), d3);
-              const R is = frcp(aa + ab);  // aa + ab >= cabs1(h) > 0
-              neg = ba * (ab * is) <= fmax(smlnum, ulp * (bb * (aa * is)));
-            }
           }
+          const bool tiny = cabs1(h) <= smlnum;
+          if (!tiny && fabs(h.re) <= ulp * tst) {
+            const cx<R> hup = ld<R>(g.at(k - 1, k));
+            const R ab = fmax(cabs1(h), cabs1(hup)), ba = fmin(cabs1(h), cabs1(hup));
+            const R d3 = cabs1(sub(hkk1, hkk));
+            const R aa = fmax(cabs1(hkk), d3), bb = fmin(cabs1(hkk), d3);
+            const R is = frcp(aa + ab);  // aa + ab >= cabs1(h) > 0
+            neg = ba * (ab * is) <= fmax(smlnum, ulp * (bb * (aa * is)));
+          }
+          neg = neg || tiny;
         }
         bal[p] = __ballot(neg);
       }
@@ -942,19 +946,12 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
     R a1 = R(0);
     if (ready) {
       cx<R> t;
-      if (its == 10) {
-        const R s = R(0.75) * fabs(ld<R>(g.at(L + 1, L)).re);
-        t = ld<R>(g.at(L, L));
-        t.re = s + t.re;
-      } else if (its == 20) {
-        const R s = R(0.75) * fabs(ld<R>(g.at(I, I - 1)).re);
-        t = ld<R>(g.at(I, I));
-        t.re = s + t.re;
-      } else {
-        t = ld<R>(g.at(I, I));
+      // the Wilkinson shift t = H(I,I) - u^2 / (x + y)
+      auto wilkinson = [&]() {
+        cx<R> tw = ld<R>(g.at(I, I));
         const cx<R> u2 = mul(ld<R>(g.at(I - 1, I)), ld<R>(g.at(I, I - 1)));
         if (u2.re != R(0) || u2.im != R(0)) {
-          const cx<R> x = scale(sub(ld<R>(g.at(I - 1, I - 1)), t), R(0.5));
+          const cx<R> x = scale(sub(ld<R>(g.at(I - 1, I - 1)), tw), R(0.5));
           const R sx = cabs1(x);
           // 2^E > max(|u|, |x|): |x / 2^E| < 1 and |u^2 / 2^2E| <= 1
           const int eu = fexp(cabs1(u2)), ex = sx > R(0) ? fexp(sx) : -4096;
@@ -964,8 +961,25 @@ __device__ __forceinline__ int lahqr(const Group<R, NB, P, LAY, LW>& g, cx<R> (&
           if (sx > R(0) && xs.re * ys.re + xs.im * ys.im < R(0)) ys = mk(-ys.re, -ys.im);
           const cx<R> d = add(xs, ys);
           const R id = frcp1(abs2(d));
-          t = sub(t, mul(cldexp(u2, -E), mk(d.re * id, -d.im * id)));
+          tw = sub(tw, mul(cldexp(u2, -E), mk(d.re * id, -d.im * id)));
         }
+        return tw;
+      };
+      // the exceptional shifts of iterations 10 and 20 behind a
+      // wavefront-uniform test, so that the usual sweep pays for no branch
+      // around them
+      if (!__builtin_expect(__any(its == 10 || its == 20), 0)) {
+        t = wilkinson();
+      } else if (its == 10) {
+        const R s = R(0.75) * fabs(ld<R>(g.at(L + 1, L)).re);
+        t = ld<R>(g.at(L, L));
+        t.re = s + t.re;
+      } else if (its == 20) {
+        const R s = R(0.75) * fabs(ld<R>(g.at(I, I - 1)).re);
+        t = ld<R>(g.at(I, I));
+        t.re = s + t.re;
+      } else {
+        t = wilkinson();
       }
       const cx<R> h11s = sub(ld<R>(g.at(L, L)), t);
       const R h21 = ld<R>(g.at(L + 1, L)).re;
