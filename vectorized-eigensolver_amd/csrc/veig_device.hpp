@@ -774,7 +774,7 @@ __device__ __forceinline__ void sweep_step(const Group<R, NB, P, LAY, LW>& g, cx
   wave_sync();
 }
 
-// The unrolled sweep over K = KEND-1 ... 0 as bit masks: bit K of the
+// The unrolled sweep over K = 0 ... KEND-1 by bit masks: bit K of the
 // wavefront-uniform kmask says some group's sweep passes step K (one scalar
 // bit test per step), bit K of the lane's amask that its own group's does
 // (one vector bit test): the range compares they replace cost 2% of the
