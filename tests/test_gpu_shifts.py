@@ -1,7 +1,8 @@
 """GPU parity where the QR iteration leaves its usual path: the exceptional
-shift of iteration 10 (zlahqr_schur, Toolbox.hpp:896-910) and the Wilkinson
-shift's u^2 = 0 case, in wavefronts whose other matrices take the usual
-shift in the same sweeps.
+shift of iteration 10 (zlahqr_schur, Toolbox.hpp:896-910), the Wilkinson
+shift's u^2 = 0 case and the deflation test's zero diagonal pair
+(Toolbox.hpp:829-843: the stalled sweeps keep the diagonal at zero), in
+wavefronts whose other matrices take the usual path in the same sweeps.
 
 The cyclic permutation matrix is unitary and already upper Hessenberg with a
 zero diagonal: H(I-1, I) = 0 makes u^2 = 0, the shift is H(I, I) = 0, and a
